@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""AMR D3Q19 benchmark: flow past a sphere on one MI355X.
+
+Workload (the single-GPU configuration of the metric): base grid 256x128x128,
+sphere of radius 16 cells centred at (64,64,64), 3 refinement levels within
+2 radii of the centre (D3Q19 neighbour-cube 2:1 balance), velocity inlet
+u = 0.05 on -x, outflow on +x, bounce-back walls elsewhere, tau0 for Re = 100
+on the diameter, regrid check every 100 steps.  A "step" is one coarse
+(level-0) time step with all level sub-cycles; MLUPS counts every leaf
+update (level l leaves update 2^l times per step).
+
+Prints one JSON line.  `value` is whole-job MLUPS; `roofline` is the level-3
+collide-stream kernel (the dominant kernel) measured with HIP events on the
+solver's stream; `cpu_baseline` is the FP64 oracle (OpenMP) on a bounded,
+64x smaller instance of the same configuration.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "AMR D3Q19 MLUPS + HBM GB/s; batched solves/s & p99 latency vs plain"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_UPDATE = {"f32": 152, "f16": 76, "f64": 304}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def sphere_params(nx0, ny0, nz0, r, u_in, re):
+    nu = u_in * 2 * r / re
+    tau0 = 3 * nu + 0.5
+    bc = ["wall", "outflow", "wall", "wall", "wall", "wall"]
+    bcu = [(u_in, 0.0, 0.0)] + [(0.0, 0.0, 0.0)] * 5
+    return tau0, bc, bcu
+
+
+def cpu_baseline(budget_s=15.0):
+    """FP64 oracle on a 1/64-size sphere instance (same shape of workload)."""
+    from oracle import oracle as O
+    from tests.amr_util import ball_region, mark_solid_ball, refine_mesh, uniform_cells
+    nx0, ny0, nz0, r = 64, 32, 32, 4.0
+    ctr = (16.0, 16.0, 16.0)
+    base = uniform_cells(O, (nx0, ny0, nz0))
+    c = refine_mesh(O, base, (nx0, ny0, nz0), [(3, ball_region(ctr, 2 * r))], 3)
+    c = mark_solid_ball(c, ctr, r)
+    tau0, bc, bcu = sphere_params(nx0, ny0, nz0, r, 0.05, 100.0)
+    bci = [{"wall": 0, "outflow": 1}[b] for b in bc]
+    s = O.Lbm(nx0, ny0, nz0, 3, tau0, bci, bcu, c.x, c.y, c.z, c.level, c.state, u_init=(0.05, 0, 0))
+    fluid = c.state == 0
+    upd = int(sum(int((fluid & (c.level == l)).sum()) << l for l in range(4)))
+    s.step(1)
+    t0 = time.perf_counter()
+    steps = 0
+    while time.perf_counter() - t0 < budget_s:
+        s.step(1)
+        steps += 1
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    return {"value": round(upd * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle, sphere {nx0}x{ny0}x{nz0} r={r:g} 3 levels ({upd} leaf updates/step), "
+                      f"{steps} steps in {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "f64"])
+    ap.add_argument("--scale", type=int, default=1, help="divide the base grid and radii by this")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--regrid-every", type=int, default=100)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    from fluidloom_amd import _lib, amr, meshes
+    from fluidloom_amd.solver import LbmSolver
+
+    L = _lib.lib()
+    _lib.check(L.fl_init(0))
+    sc = args.scale
+    nx0, ny0, nz0 = 256 // sc, 128 // sc, 128 // sc
+    r = 16.0 / sc
+    ctr = (64.0 / sc, 64.0 / sc, 64.0 / sc)
+    u_in, re, levels = 0.05, 100.0, 3
+    tau0, bc, bcu = sphere_params(nx0, ny0, nz0, r, u_in, re)
+
+    t0 = time.perf_counter()
+    mesh = meshes.sphere(nx0, ny0, nz0, ctr, r, 2 * r, levels, log=log)
+    t1 = time.perf_counter()
+    log(f"mesh built on device in {t1 - t0:.1f} s")
+    solver = LbmSolver(nx0, ny0, nz0, levels, tau0, bc, bcu, mesh["x"], mesh["y"], mesh["z"], mesh["level"],
+                       mesh["state"], storage=args.storage, u_init=(u_in, 0.0, 0.0))
+    t2 = time.perf_counter()
+    st = solver.stats()
+    leaves = [int(st.leaves[l]) for l in range(levels + 1)]
+    upd = int(st.updates_per_step)
+    log(f"solver layout built in {t2 - t1:.1f} s: leaves/level {leaves}, bricks/level "
+        f"{[int(st.bricks[l]) for l in range(levels + 1)]}, {upd} updates/step, "
+        f"{st.device_bytes / 2**30:.2f} GiB")
+    # leaves on the device for the periodic regrid check
+    cells = amr.GpuCells.from_host(**{k: mesh[k] for k in ("x", "y", "z", "level", "state")})
+
+    def regrid_check():
+        return amr.mark_ball(cells, ctr, 2 * r, levels)
+
+    solver.step(args.warmup)
+    solver.sync()
+    ev0, ev1 = _lib.C.c_void_p(), _lib.C.c_void_p()
+    stream = _lib.C.c_void_p()
+    _lib.check(L.fl_lbm_stream(solver.h, _lib.C.byref(stream)))
+    _lib.check(L.fl_event_create(_lib.C.byref(ev0)))
+    _lib.check(L.fl_event_create(_lib.C.byref(ev1)))
+    _lib.check(L.fl_device_sync())
+    regrids = 0
+    w0 = time.perf_counter()
+    _lib.check(L.fl_event_record(ev0, stream))
+    for k in range(args.steps):
+        if args.regrid_every and k > 0 and k % args.regrid_every == 0:
+            if regrid_check() != 0:
+                raise RuntimeError("sphere refinement changed; static mesh expected")
+            regrids += 1
+        solver.step(1)
+        if k % 25 == 24:
+            log(f"step {k + 1}/{args.steps}")
+    _lib.check(L.fl_event_record(ev1, stream))
+    _lib.check(L.fl_device_sync())
+    w1 = time.perf_counter()
+    ms = _lib.C.c_float(0)
+    _lib.check(L.fl_event_elapsed_ms(ev0, ev1, _lib.C.byref(ms)))
+    wall = w1 - w0
+    mlups = upd * args.steps / wall / 1e6
+    bpu = BYTES_PER_UPDATE[args.storage]
+    F = solver.force()
+    cd = F[0] / (0.5 * 1.0 * u_in ** 2 * np.pi * r ** 2)
+    mass, mom = solver.totals()
+
+    # roofline of the dominant kernel: level-3 collide-stream alone
+    iters = 20
+    ms_k = solver.time_collide(levels, iters)
+    alg_bytes = leaves[levels] * bpu
+    achieved = alg_bytes / (ms_k * 1e-3) / 1e9
+    log(f"level-{levels} collide: {ms_k:.3f} ms/launch, {achieved:.0f} GB/s algorithmic")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle, bounded sample)...")
+        cpu = cpu_baseline()
+
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.storage,
+        "data": "synthetic",
+        "config": {"workload": f"flow past a sphere: base {nx0}x{ny0}x{nz0}, r={r:g} at {ctr}, "
+                               f"{levels} levels within {2 * r:g} cells, Re={re:g}, u_in={u_in}, "
+                               f"tau0={tau0:.4f}, regrid check every {args.regrid_every} steps",
+                   "leaves_per_level": leaves, "updates_per_step": upd, "storage": args.storage,
+                   "parallelism": "single GPU"},
+        "hbm_gbps_algorithmic": round(mlups * 1e6 * bpu / 1e9, 1),
+        "gpu_event_ms": round(ms.value, 3),
+        "regrid_checks": regrids,
+        "drag_coefficient": round(float(cd), 4),
+        "mass": mass,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": f"k_collide level {levels} ({leaves[levels]} leaves x {bpu} B)"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,154 @@
+// Device, memory, stream and error plumbing of the C ABI.
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "fl_internal.hpp"
+
+namespace fl {
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+    g_last_error = buf;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return FL_E_OOM;
+    return FL_E_HIP;
+}
+}  // namespace fl
+
+extern "C" {
+
+const char* fl_last_error_string(void) { return fl::g_last_error.c_str(); }
+
+const char* fl_version(void) { return "fluidloom-hip 0.1 (gfx950)"; }
+
+fl_status fl_init(int device) {
+    int n = 0;
+    FL_HIP(hipGetDeviceCount(&n));
+    FL_REQUIRE(device >= 0 && device < n, "device id out of range");
+    FL_HIP(hipSetDevice(device));
+    FL_HIP(hipFree(nullptr));
+    return FL_OK;
+}
+
+fl_status fl_device_count(int* count) {
+    FL_REQUIRE(count != nullptr, "null count");
+    hipError_t e = hipGetDeviceCount(count);
+    if (e == hipErrorNoDevice) { *count = 0; (void)hipGetLastError(); return FL_OK; }
+    FL_HIP(e);
+    return FL_OK;
+}
+
+fl_status fl_device_info(int device, char* name, size_t name_len, size_t* total_mem, int* cu_count,
+                         int* gcn_arch) {
+    hipDeviceProp_t p;
+    FL_HIP(hipGetDeviceProperties(&p, device));
+    if (name && name_len) {
+        std::strncpy(name, p.gcnArchName, name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (total_mem) *total_mem = p.totalGlobalMem;
+    if (cu_count) *cu_count = p.multiProcessorCount;
+    if (gcn_arch) *gcn_arch = p.major * 100 + p.minor * 10;
+    return FL_OK;
+}
+
+fl_status fl_malloc(void** d_ptr, size_t bytes) {
+    FL_REQUIRE(d_ptr != nullptr, "null output pointer");
+    FL_REQUIRE(bytes > 0, "zero-size allocation");
+    FL_HIP(hipMalloc(d_ptr, bytes));
+    return FL_OK;
+}
+
+fl_status fl_free(void* d_ptr) {
+    if (!d_ptr) return FL_OK;
+    FL_HIP(hipFree(d_ptr));
+    return FL_OK;
+}
+
+fl_status fl_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes, fl_stream s) {
+    if (bytes == 0) return FL_OK;
+    FL_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, FL_STREAM(s)));
+    FL_HIP(hipStreamSynchronize(FL_STREAM(s)));
+    return FL_OK;
+}
+
+fl_status fl_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes, fl_stream s) {
+    if (bytes == 0) return FL_OK;
+    FL_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, FL_STREAM(s)));
+    FL_HIP(hipStreamSynchronize(FL_STREAM(s)));
+    return FL_OK;
+}
+
+fl_status fl_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes, fl_stream s) {
+    if (bytes == 0) return FL_OK;
+    FL_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, FL_STREAM(s)));
+    return FL_OK;
+}
+
+fl_status fl_memset(void* d_dst, int value, size_t bytes, fl_stream s) {
+    if (bytes == 0) return FL_OK;
+    FL_HIP(hipMemsetAsync(d_dst, value, bytes, FL_STREAM(s)));
+    return FL_OK;
+}
+
+fl_status fl_stream_create(fl_stream* s) {
+    FL_REQUIRE(s != nullptr, "null stream pointer");
+    hipStream_t h;
+    FL_HIP(hipStreamCreateWithFlags(&h, hipStreamNonBlocking));
+    *s = reinterpret_cast<fl_stream>(h);
+    return FL_OK;
+}
+
+fl_status fl_stream_destroy(fl_stream s) {
+    if (!s) return FL_OK;
+    FL_HIP(hipStreamDestroy(FL_STREAM(s)));
+    return FL_OK;
+}
+
+fl_status fl_stream_sync(fl_stream s) {
+    FL_HIP(hipStreamSynchronize(FL_STREAM(s)));
+    return FL_OK;
+}
+
+fl_status fl_device_sync(void) {
+    FL_HIP(hipDeviceSynchronize());
+    return FL_OK;
+}
+
+fl_status fl_event_create(fl_event* e) {
+    FL_REQUIRE(e != nullptr, "null event pointer");
+    hipEvent_t h;
+    FL_HIP(hipEventCreate(&h));
+    *e = reinterpret_cast<fl_event>(h);
+    return FL_OK;
+}
+
+fl_status fl_event_destroy(fl_event e) {
+    if (!e) return FL_OK;
+    FL_HIP(hipEventDestroy(reinterpret_cast<hipEvent_t>(e)));
+    return FL_OK;
+}
+
+fl_status fl_event_record(fl_event e, fl_stream s) {
+    FL_HIP(hipEventRecord(reinterpret_cast<hipEvent_t>(e), FL_STREAM(s)));
+    return FL_OK;
+}
+
+fl_status fl_event_elapsed_ms(fl_event a, fl_event b, float* ms) {
+    FL_HIP(hipEventSynchronize(reinterpret_cast<hipEvent_t>(b)));
+    FL_HIP(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(a), reinterpret_cast<hipEvent_t>(b)));
+    return FL_OK;
+}
+
+fl_status fl_mem_info(size_t* free_bytes, size_t* total_bytes) {
+    FL_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    return FL_OK;
+}
+
+}  // extern "C"

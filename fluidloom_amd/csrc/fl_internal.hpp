@@ -1,0 +1,45 @@
+// Internal helpers shared by every translation unit of libfluidloom_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "fluidloom_hip.h"
+
+namespace fl {
+
+// Records the message returned by fl_last_error_string() for this thread.
+void set_error(const std::string& msg);
+// Records a HIP failure; returns FL_E_HIP (or FL_E_OOM for out-of-memory).
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+constexpr uint64_t kEmptyKey = 0xFFFFFFFFFFFFFFFFull;
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+constexpr uint32_t kFiner = 0xFFFFFFFEu;
+
+inline unsigned grid_for(size_t n, unsigned block) {
+    size_t g = (n + block - 1) / block;
+    return (unsigned)(g == 0 ? 1 : g);
+}
+
+}  // namespace fl
+
+#define FL_HIP(call)                                                        \
+    do {                                                                    \
+        hipError_t fl_e_ = (call);                                          \
+        if (fl_e_ != hipSuccess) return fl::hip_fail(fl_e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define FL_CHECK_LAUNCH() FL_HIP(hipGetLastError())
+
+#define FL_REQUIRE(cond, msg)                                               \
+    do {                                                                    \
+        if (!(cond)) {                                                      \
+            fl::set_error(std::string(__func__) + ": " + (msg));            \
+            return FL_E_INVALID;                                            \
+        }                                                                   \
+    } while (0)
+
+#define FL_STREAM(s) (reinterpret_cast<hipStream_t>(s))

@@ -1,0 +1,73 @@
+// a5: single-pass exclusive scan of u32 with decoupled look-back.
+//
+// Replaces the reference's three-launch Blelloch scan
+// (src/adaptation/kernels/compact_cells.cl:61-125), which has no carry above
+// its second level and is wrong for n > 65,536.  A workgroup takes the next
+// 4096-element tile (dynamic id), scans 16 consecutive elements per thread,
+// scans the 256 thread totals with wave shuffles, publishes its tile total
+// and resolves its prefix by look-back over 64-bit (flag, value) words.
+#include "fl_internal.hpp"
+#include "scan_device.hpp"
+#include "wave.hpp"
+
+namespace fl {
+namespace {
+
+__global__ void k_zero64(uint64_t* __restrict__ p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan(const uint32_t* __restrict__ in,
+                                                       uint32_t* __restrict__ out,
+                                                       uint64_t* __restrict__ total, size_t n,
+                                                       uint64_t* __restrict__ ws) {
+    __shared__ ScanShared sh;
+    const uint32_t tile = scan_tile_id(sh, ws);
+    const size_t base = (size_t)tile * kScanTile + (size_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint64_t acc = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        size_t i = base + j;
+        v[j] = i < n ? in[i] : 0u;
+        acc += v[j];
+    }
+    uint64_t excl = scan_tile_prefix(sh, ws, tile, acc, total, n);
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        size_t i = base + j;
+        if (i < n) out[i] = (uint32_t)excl;
+        excl += v[j];
+    }
+}
+
+}  // namespace
+}  // namespace fl
+
+using namespace fl;
+
+extern "C" {
+
+size_t fl_scan_temp_bytes(size_t n) { return scan_ws_words(n) * sizeof(uint64_t); }
+
+fl_status fl_scan_exclusive_u32(const uint32_t* d_in, uint32_t* d_out, uint64_t* d_total, size_t n,
+                                void* d_temp, size_t temp_bytes, fl_stream s) {
+    hipStream_t st = FL_STREAM(s);
+    if (n == 0) {
+        if (d_total) FL_HIP(hipMemsetAsync(d_total, 0, sizeof(uint64_t), st));
+        return FL_OK;
+    }
+    FL_REQUIRE(d_in && d_out && d_temp, "null pointer");
+    size_t words = scan_ws_words(n);
+    FL_REQUIRE(temp_bytes >= words * sizeof(uint64_t), "temp buffer too small (see fl_scan_temp_bytes)");
+    uint64_t* ws = static_cast<uint64_t*>(d_temp);
+    hipLaunchKernelGGL(k_zero64, dim3(grid_for(words, 256) > 2048 ? 2048 : grid_for(words, 256)), dim3(256), 0,
+                       st, ws, words);
+    FL_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_scan, dim3(scan_tiles(n)), dim3(kScanThreads), 0, st, d_in, d_out, d_total, n, ws);
+    FL_CHECK_LAUNCH();
+    return FL_OK;
+}
+
+}  // extern "C"

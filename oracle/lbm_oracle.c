@@ -1,0 +1,470 @@
+/*
+ * lbm_oracle.c -- FP64 CPU oracle of the AMR D3Q19 step (test infrastructure
+ * only; never linked into the product library).  See lbm_oracle.h.
+ *
+ * Scheme (the build's specification; the reference has no LBM arithmetic):
+ *  - D3Q19, velocities ordered rest, +-x, +-y, +-z (the order of the lattice
+ *    fixture in tests/unit/parsing/test_parsing.cpp:23-29), then the 12 edge
+ *    directions in opposite pairs; opposites found as in
+ *    LatticeDescriptor::computeOpposites (LatticeDescriptor.cpp:30-49).
+ *  - Stored state: post-collision populations f*; pull streaming
+ *    f_i(x) = f*_i(x - c_i); BGK f* = f - (f - feq)/tau.
+ *  - Walls: half-way bounce-back f_i(x) = f*_opp(x) + 6 w_i rho0 (c_i.u_w)
+ *    (Ladd); outflow faces copy from the position clamped into the domain.
+ *  - Level l: dx = 2^-l, dt = 2^-l, tau_l = 2^l (tau0 - 1/2) + 1/2.
+ *  - Coarse step advance(l): f2c fill, capture pre-collision state of c2f
+ *    sources (from the previous buffer), collide-stream, capture again, then
+ *    two sub-steps of level l+1 each preceded by a c2f fill at alpha = 0, 1/2.
+ *  - c2f: f_fine* = feq(rho,u) + (1-1/tau_f) (tau_f/(2 tau_c)) fneq_c.
+ *  - f2c: average rho, j, fneq of the 8 children's pre-collision states;
+ *    f_coarse* = feq + (1-1/tau_c) (2 tau_c/tau_f) avg fneq.
+ */
+#include "lbm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define Q 19
+enum { T_NONE = 0, T_FLUID = 1, T_SOLID = 2, T_C2F = 3, T_F2C = 4, T_WALL = 5 };
+
+static const int CX[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+static const int CY[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+static const int CZ[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+static double W[Q];
+static int OPP[Q];
+static int lattice_ready = 0;
+
+static void init_lattice(void) {
+    if (lattice_ready) return;
+    for (int i = 0; i < Q; ++i) {
+        int n = abs(CX[i]) + abs(CY[i]) + abs(CZ[i]);
+        W[i] = n == 0 ? 1.0 / 3.0 : (n == 1 ? 1.0 / 18.0 : 1.0 / 36.0);
+        for (int j = 0; j < Q; ++j)
+            if (CX[i] == -CX[j] && CY[i] == -CY[j] && CZ[i] == -CZ[j]) { OPP[i] = j; break; }
+    }
+    lattice_ready = 1;
+}
+
+void or_lbm_lattice(int* c, double* w, int* opp) {
+    init_lattice();
+    for (int i = 0; i < Q; ++i) {
+        c[3 * i] = CX[i]; c[3 * i + 1] = CY[i]; c[3 * i + 2] = CZ[i];
+        w[i] = W[i]; opp[i] = OPP[i];
+    }
+}
+
+typedef struct {
+    int X, Y, Z;
+    size_t N;
+    uint8_t* type;
+    int64_t* c2f_src;   /* coarse position for T_C2F */
+    double* f[2];       /* [Q][N] */
+    double* cap[2];     /* [Q][N] at capture cells */
+    int cur;
+    double tau;
+    size_t nfluid, nc2f, nf2c, ncap;
+    int64_t* fluid;     /* positions */
+    int64_t* c2f;
+    int64_t* f2c;
+    int64_t* capl;
+    uint8_t* is_cap;
+} level_t;
+
+struct or_lbm {
+    or_lbm_params p;
+    int L;              /* number of levels */
+    level_t lv[24];
+    double c2f_mult, f2c_mult; /* 1.0; tests plant mistakes through these */
+};
+
+static inline int64_t pidx(const level_t* v, int x, int y, int z) {
+    return ((int64_t)z * v->Y + y) * v->X + x;
+}
+static inline void pxyz(const level_t* v, int64_t p, int* x, int* y, int* z) {
+    *x = (int)(p % v->X);
+    *y = (int)((p / v->X) % v->Y);
+    *z = (int)(p / ((int64_t)v->X * v->Y));
+}
+static inline int regular(int t) { return t == T_FLUID || t == T_C2F || t == T_F2C; }
+
+static inline double feq(double rho, double ux, double uy, double uz, int i) {
+    double cu = CX[i] * ux + CY[i] * uy + CZ[i] * uz;
+    double usq = ux * ux + uy * uy + uz * uz;
+    return W[i] * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq);
+}
+
+static inline void moments(const double* f, double* rho, double* ux, double* uy, double* uz) {
+    double r = 0, jx = 0, jy = 0, jz = 0;
+    for (int i = 0; i < Q; ++i) {
+        r += f[i];
+        jx += CX[i] * f[i]; jy += CY[i] * f[i]; jz += CZ[i] * f[i];
+    }
+    *rho = r; *ux = jx / r; *uy = jy / r; *uz = jz / r;
+}
+
+/* Source of direction i for position (x,y,z) after periodic wrap.  Returns
+ * 0 inside, else 1 and the wall face (or -1 for outflow) in *face. */
+static int source(const or_lbm* s, const level_t* v, int x, int y, int z, int i, int* sx, int* sy,
+                  int* sz, int* face) {
+    int p[3] = {x - CX[i], y - CY[i], z - CZ[i]};
+    int e[3] = {v->X, v->Y, v->Z};
+    int out[3] = {0, 0, 0};
+    int any = 0;
+    for (int a = 0; a < 3; ++a) {
+        int f = -1;
+        if (p[a] < 0) f = 2 * a;
+        else if (p[a] >= e[a]) f = 2 * a + 1;
+        if (f < 0) continue;
+        if (s->p.bc[f] == OR_BC_PERIODIC) {
+            p[a] = (p[a] + e[a]) % e[a];
+        } else {
+            out[a] = f + 1;
+            any = 1;
+        }
+    }
+    *sx = p[0]; *sy = p[1]; *sz = p[2];
+    if (!any) return 0;
+    *face = -1;
+    for (int a = 2; a >= 0; --a)
+        if (out[a] && s->p.bc[out[a] - 1] == OR_BC_WALL) { *face = out[a] - 1; break; }
+    if (*face < 0) {
+        /* outflow: clamp the outside axes */
+        for (int a = 0; a < 3; ++a)
+            if (out[a]) p[a] = p[a] < 0 ? 0 : e[a] - 1;
+        *sx = p[0]; *sy = p[1]; *sz = p[2];
+    }
+    return 1;
+}
+
+static void gather(const or_lbm* s, const level_t* v, int64_t pos, const double* buf, double* out) {
+    int x, y, z;
+    pxyz(v, pos, &x, &y, &z);
+    const size_t N = v->N;
+    for (int i = 0; i < Q; ++i) {
+        int sx, sy, sz, face;
+        int o = source(s, v, x, y, z, i, &sx, &sy, &sz, &face);
+        if (o && face >= 0) {
+            const double* uw = s->p.bc_u[face];
+            out[i] = buf[(size_t)OPP[i] * N + pos] +
+                     6.0 * W[i] * s->p.rho0 * (CX[i] * uw[0] + CY[i] * uw[1] + CZ[i] * uw[2]);
+            continue;
+        }
+        int64_t sp = pidx(v, sx, sy, sz);
+        if (regular(v->type[sp])) out[i] = buf[(size_t)i * N + sp];
+        else out[i] = buf[(size_t)OPP[i] * N + pos];
+    }
+}
+
+static void collide(const double* f, double tau, double* o) {
+    double rho, ux, uy, uz;
+    moments(f, &rho, &ux, &uy, &uz);
+    for (int i = 0; i < Q; ++i) o[i] = f[i] - (f[i] - feq(rho, ux, uy, uz, i)) / tau;
+}
+
+static int classify(or_lbm* s, int l, int64_t p) {
+    level_t* v = &s->lv[l];
+    if (v->type[p] != T_NONE) return 0;
+    int x, y, z;
+    pxyz(v, p, &x, &y, &z);
+    if (l > 0) {
+        level_t* c = &s->lv[l - 1];
+        int64_t cp = pidx(c, x >> 1, y >> 1, z >> 1);
+        if (c->type[cp] == T_FLUID) {
+            v->type[p] = T_C2F;
+            v->c2f_src[p] = cp;
+            c->is_cap[cp] = 1;
+            return 0;
+        }
+        if (c->type[cp] == T_SOLID) { v->type[p] = T_WALL; return 0; }
+    }
+    if (l + 1 < s->L) {
+        level_t* f = &s->lv[l + 1];
+        int nf = 0, ns = 0;
+        for (int q = 0; q < 8; ++q) {
+            int t = f->type[pidx(f, 2 * x + (q & 1), 2 * y + ((q >> 1) & 1), 2 * z + ((q >> 2) & 1))];
+            nf += t == T_FLUID;
+            ns += t == T_SOLID;
+        }
+        if (nf == 8) { v->type[p] = T_F2C; return 0; }
+        if (nf + ns == 8) { v->type[p] = T_WALL; return 0; }
+    }
+    return -1;
+}
+
+or_lbm* or_lbm_create(const or_lbm_params* p, const int32_t* x, const int32_t* y,
+                      const int32_t* z, const uint8_t* level, const uint8_t* state, size_t n,
+                      double rho_init, const double u_init[3], int* err) {
+    init_lattice();
+    *err = 0;
+    or_lbm* s = (or_lbm*)calloc(1, sizeof(or_lbm));
+    s->p = *p;
+    s->L = p->max_level + 1;
+    s->c2f_mult = 1.0;
+    s->f2c_mult = 1.0;
+    for (int l = 0; l < s->L; ++l) {
+        level_t* v = &s->lv[l];
+        v->X = p->nx0 << l; v->Y = p->ny0 << l; v->Z = p->nz0 << l;
+        v->N = (size_t)v->X * v->Y * v->Z;
+        v->type = (uint8_t*)calloc(v->N, 1);
+        v->is_cap = (uint8_t*)calloc(v->N, 1);
+        v->c2f_src = (int64_t*)malloc(v->N * sizeof(int64_t));
+        v->tau = ldexp(p->tau0 - 0.5, l) + 0.5;
+    }
+    for (size_t k = 0; k < n; ++k) {
+        int l = level[k];
+        if (l >= s->L) { *err = 1; or_lbm_destroy(s); return NULL; }
+        level_t* v = &s->lv[l];
+        if (x[k] < 0 || y[k] < 0 || z[k] < 0 || x[k] >= v->X || y[k] >= v->Y || z[k] >= v->Z) {
+            *err = 2; or_lbm_destroy(s); return NULL;
+        }
+        int64_t q = pidx(v, x[k], y[k], z[k]);
+        if (v->type[q] != T_NONE) { *err = 3; or_lbm_destroy(s); return NULL; }
+        v->type[q] = state[k] == 0 ? T_FLUID : T_SOLID;
+    }
+    /* classify every source of every fluid leaf */
+    for (int l = 0; l < s->L; ++l) {
+        level_t* v = &s->lv[l];
+        for (int64_t q = 0; q < (int64_t)v->N; ++q) {
+            if (v->type[q] != T_FLUID) continue;
+            int xx, yy, zz;
+            pxyz(v, q, &xx, &yy, &zz);
+            for (int i = 1; i < Q; ++i) {
+                int sx, sy, sz, face;
+                int o = source(s, v, xx, yy, zz, i, &sx, &sy, &sz, &face);
+                if (o && face >= 0) continue;
+                if (classify(s, l, pidx(v, sx, sy, sz)) != 0) {
+                    *err = 4; or_lbm_destroy(s); return NULL;
+                }
+            }
+        }
+    }
+    for (int l = 0; l < s->L; ++l) {
+        level_t* v = &s->lv[l];
+        size_t nf = 0, nc = 0, nd = 0, np = 0;
+        for (size_t q = 0; q < v->N; ++q) {
+            nf += v->type[q] == T_FLUID;
+            nc += v->type[q] == T_C2F;
+            nd += v->type[q] == T_F2C;
+            np += v->is_cap[q];
+        }
+        v->fluid = (int64_t*)malloc((nf + 1) * sizeof(int64_t));
+        v->c2f = (int64_t*)malloc((nc + 1) * sizeof(int64_t));
+        v->f2c = (int64_t*)malloc((nd + 1) * sizeof(int64_t));
+        v->capl = (int64_t*)malloc((np + 1) * sizeof(int64_t));
+        v->nfluid = v->nc2f = v->nf2c = v->ncap = 0;
+        for (size_t q = 0; q < v->N; ++q) {
+            if (v->type[q] == T_FLUID) v->fluid[v->nfluid++] = (int64_t)q;
+            if (v->type[q] == T_C2F) v->c2f[v->nc2f++] = (int64_t)q;
+            if (v->type[q] == T_F2C) v->f2c[v->nf2c++] = (int64_t)q;
+            if (v->is_cap[q]) v->capl[v->ncap++] = (int64_t)q;
+        }
+        for (int b = 0; b < 2; ++b) {
+            v->f[b] = (double*)calloc((size_t)Q * v->N, sizeof(double));
+            v->cap[b] = (double*)calloc((size_t)Q * v->N, sizeof(double));
+        }
+        for (size_t q = 0; q < v->N; ++q) {
+            if (!regular(v->type[q])) continue;
+            for (int i = 0; i < Q; ++i) {
+                double e = feq(rho_init, u_init[0], u_init[1], u_init[2], i);
+                v->f[0][(size_t)i * v->N + q] = e;
+                v->f[1][(size_t)i * v->N + q] = e;
+            }
+        }
+        v->cur = 0;
+    }
+    return s;
+}
+
+void or_lbm_destroy(or_lbm* s) {
+    if (!s) return;
+    for (int l = 0; l < s->L; ++l) {
+        level_t* v = &s->lv[l];
+        free(v->type); free(v->is_cap); free(v->c2f_src);
+        free(v->f[0]); free(v->f[1]); free(v->cap[0]); free(v->cap[1]);
+        free(v->fluid); free(v->c2f); free(v->f2c); free(v->capl);
+    }
+    free(s);
+}
+
+static void fill_f2c(or_lbm* s, int l) {
+    level_t* v = &s->lv[l];
+    level_t* f = &s->lv[l + 1];
+    const double* fprev = f->f[1 - f->cur];
+    double* dst = v->f[v->cur];
+    const double fac = (1.0 - 1.0 / v->tau) * (2.0 * v->tau / f->tau) * s->f2c_mult;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < (int64_t)v->nf2c; ++k) {
+        int64_t g = v->f2c[k];
+        int x, y, z;
+        pxyz(v, g, &x, &y, &z);
+        double sr = 0, sjx = 0, sjy = 0, sjz = 0, sneq[Q];
+        for (int i = 0; i < Q; ++i) sneq[i] = 0;
+        for (int q = 0; q < 8; ++q) {
+            int64_t cp = pidx(f, 2 * x + (q & 1), 2 * y + ((q >> 1) & 1), 2 * z + ((q >> 2) & 1));
+            double fk[Q], r, ux, uy, uz;
+            gather(s, f, cp, fprev, fk);
+            moments(fk, &r, &ux, &uy, &uz);
+            sr += r; sjx += r * ux; sjy += r * uy; sjz += r * uz;
+            for (int i = 0; i < Q; ++i) sneq[i] += fk[i] - feq(r, ux, uy, uz, i);
+        }
+        double r = sr * 0.125;
+        double ux = (sjx * 0.125) / r, uy = (sjy * 0.125) / r, uz = (sjz * 0.125) / r;
+        for (int i = 0; i < Q; ++i)
+            dst[(size_t)i * v->N + g] = feq(r, ux, uy, uz, i) + fac * (sneq[i] * 0.125);
+    }
+}
+
+static void capture(or_lbm* s, int l, int slot) {
+    level_t* v = &s->lv[l];
+    const double* prev = v->f[1 - v->cur];
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < (int64_t)v->ncap; ++k) {
+        int64_t c = v->capl[k];
+        double fk[Q];
+        gather(s, v, c, prev, fk);
+        for (int i = 0; i < Q; ++i) v->cap[slot][(size_t)i * v->N + c] = fk[i];
+    }
+}
+
+static void fill_c2f(or_lbm* s, int l, double alpha) {
+    level_t* v = &s->lv[l];
+    level_t* c = &s->lv[l - 1];
+    double* dst = v->f[v->cur];
+    const double fac = (1.0 - 1.0 / v->tau) * (v->tau / (2.0 * c->tau)) * s->c2f_mult;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < (int64_t)v->nc2f; ++k) {
+        int64_t g = v->c2f[k];
+        int64_t cp = v->c2f_src[g];
+        double fp[Q], r, ux, uy, uz;
+        for (int i = 0; i < Q; ++i)
+            fp[i] = (1.0 - alpha) * c->cap[0][(size_t)i * c->N + cp] + alpha * c->cap[1][(size_t)i * c->N + cp];
+        moments(fp, &r, &ux, &uy, &uz);
+        for (int i = 0; i < Q; ++i) {
+            double e = feq(r, ux, uy, uz, i);
+            dst[(size_t)i * v->N + g] = e + fac * (fp[i] - e);
+        }
+    }
+}
+
+static void collide_stream(or_lbm* s, int l) {
+    level_t* v = &s->lv[l];
+    const double* src = v->f[v->cur];
+    double* dst = v->f[1 - v->cur];
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < (int64_t)v->nfluid; ++k) {
+        int64_t p = v->fluid[k];
+        double fk[Q], o[Q];
+        gather(s, v, p, src, fk);
+        collide(fk, v->tau, o);
+        for (int i = 0; i < Q; ++i) dst[(size_t)i * v->N + p] = o[i];
+    }
+    v->cur = 1 - v->cur;
+}
+
+static void advance(or_lbm* s, int l) {
+    int fine = l + 1 < s->L;
+    if (fine) {
+        fill_f2c(s, l);
+        capture(s, l, 0);
+    }
+    collide_stream(s, l);
+    if (fine) {
+        capture(s, l, 1);
+        for (int sub = 0; sub < 2; ++sub) {
+            fill_c2f(s, l + 1, 0.5 * sub);
+            advance(s, l + 1);
+        }
+    }
+}
+
+void or_lbm_step(or_lbm* s, int nsteps) {
+    for (int t = 0; t < nsteps; ++t) advance(s, 0);
+}
+
+static int64_t leaf_pos(const or_lbm* s, int32_t x, int32_t y, int32_t z, int l) {
+    if (l >= s->L) return -1;
+    const level_t* v = &s->lv[l];
+    if (x < 0 || y < 0 || z < 0 || x >= v->X || y >= v->Y || z >= v->Z) return -1;
+    return pidx(v, x, y, z);
+}
+
+void or_lbm_get_f(const or_lbm* s, const int32_t* x, const int32_t* y, const int32_t* z,
+                  const uint8_t* level, size_t n, double* f) {
+    for (size_t k = 0; k < n; ++k) {
+        int64_t p = leaf_pos(s, x[k], y[k], z[k], level[k]);
+        const level_t* v = &s->lv[level[k]];
+        for (int i = 0; i < Q; ++i)
+            f[k * Q + i] = p < 0 ? NAN : v->f[v->cur][(size_t)i * v->N + p];
+    }
+}
+
+void or_lbm_set_f(or_lbm* s, const int32_t* x, const int32_t* y, const int32_t* z,
+                  const uint8_t* level, size_t n, const double* f) {
+    for (size_t k = 0; k < n; ++k) {
+        int64_t p = leaf_pos(s, x[k], y[k], z[k], level[k]);
+        if (p < 0) continue;
+        level_t* v = &s->lv[level[k]];
+        for (int i = 0; i < Q; ++i) {
+            v->f[0][(size_t)i * v->N + p] = f[k * Q + i];
+            v->f[1][(size_t)i * v->N + p] = f[k * Q + i];
+        }
+    }
+}
+
+void or_lbm_moments(const or_lbm* s, const int32_t* x, const int32_t* y, const int32_t* z,
+                    const uint8_t* level, size_t n, double* rho, double* u) {
+    for (size_t k = 0; k < n; ++k) {
+        int64_t p = leaf_pos(s, x[k], y[k], z[k], level[k]);
+        if (p < 0) { rho[k] = NAN; u[3 * k] = u[3 * k + 1] = u[3 * k + 2] = NAN; continue; }
+        const level_t* v = &s->lv[level[k]];
+        double f[Q];
+        for (int i = 0; i < Q; ++i) f[i] = v->f[v->cur][(size_t)i * v->N + p];
+        moments(f, &rho[k], &u[3 * k], &u[3 * k + 1], &u[3 * k + 2]);
+    }
+}
+
+void or_lbm_totals(const or_lbm* s, double* mass, double mom[3]) {
+    *mass = 0; mom[0] = mom[1] = mom[2] = 0;
+    for (int l = 0; l < s->L; ++l) {
+        const level_t* v = &s->lv[l];
+        double vol = ldexp(1.0, -3 * l);
+        for (size_t k = 0; k < v->nfluid; ++k) {
+            int64_t p = v->fluid[k];
+            double f[Q], r, ux, uy, uz;
+            for (int i = 0; i < Q; ++i) f[i] = v->f[v->cur][(size_t)i * v->N + p];
+            moments(f, &r, &ux, &uy, &uz);
+            *mass += r * vol;
+            mom[0] += r * ux * vol; mom[1] += r * uy * vol; mom[2] += r * uz * vol;
+        }
+    }
+}
+
+void or_lbm_force(const or_lbm* s, double F[3]) {
+    F[0] = F[1] = F[2] = 0;
+    for (int l = 0; l < s->L; ++l) {
+        const level_t* v = &s->lv[l];
+        double fx = 0, fy = 0, fz = 0;
+        const double* b = v->f[v->cur];
+        for (size_t k = 0; k < v->nfluid; ++k) {
+            int64_t p = v->fluid[k];
+            int x, y, z;
+            pxyz(v, p, &x, &y, &z);
+            for (int i = 1; i < Q; ++i) {
+                int sx, sy, sz, face;
+                if (source(s, v, x, y, z, i, &sx, &sy, &sz, &face)) continue;
+                int t = v->type[pidx(v, sx, sy, sz)];
+                if (t != T_SOLID && t != T_WALL) continue;
+                double fo = b[(size_t)OPP[i] * v->N + p];
+                fx += -2.0 * CX[i] * fo; fy += -2.0 * CY[i] * fo; fz += -2.0 * CZ[i] * fo;
+            }
+        }
+        double sc = ldexp(1.0, -2 * l);
+        F[0] += fx * sc; F[1] += fy * sc; F[2] += fz * sc;
+    }
+}
+
+void or_lbm_plant_coupling_error(or_lbm* s, double c2f_mult, double f2c_mult) {
+    s->c2f_mult = c2f_mult;
+    s->f2c_mult = f2c_mult;
+}

@@ -1,0 +1,185 @@
+"""GPU parity of the integer hot path (keys, sort, scan, hash, neighbours,
+adaptation) against the oracle: bit-exact, through the C ABI."""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden" / "hilbert_golden.npz"
+
+
+def dev(fl, a):
+    return fl.DeviceArray.from_host(np.ascontiguousarray(a))
+
+
+def test_hilbert_encode_decode_golden(fl):
+    g = np.load(GOLD)
+    n = len(g["x"])
+    x, y, z, lv = dev(fl, g["x"]), dev(fl, g["y"]), dev(fl, g["z"]), dev(fl, g["level"])
+    out = fl.DeviceArray(n, np.uint64)
+    fl.check(fl.lib().fl_hilbert_encode(x.p, y.p, z.p, lv.p, 0, out.p, n, None))
+    assert np.array_equal(out.to_host(), g["encode"])
+    k = dev(fl, g["encode"])
+    ox, oy, oz = (fl.DeviceArray(n, np.int32) for _ in range(3))
+    fl.check(fl.lib().fl_hilbert_decode(k.p, lv.p, 0, ox.p, oy.p, oz.p, n, None))
+    assert np.array_equal(ox.to_host(), g["dec_x"]) and np.array_equal(oy.to_host(), g["dec_y"])
+    assert np.array_equal(oz.to_host(), g["dec_z"])
+    for ml, name in [(8, "rotdir8"), (10, "rotdir10")]:
+        fl.check(fl.lib().fl_rotdir_encode(x.p, y.p, z.p, ml, out.p, n, None))
+        assert np.array_equal(out.to_host(), g[name])
+
+
+def test_hilbert_scalar_level_and_key63(fl, oracle):
+    rng = np.random.default_rng(1)
+    for n in [1, 63, 1000, 100003]:
+        for level in [0, 1, 7, 10, 21]:
+            x, y, z = (rng.integers(0, 1 << max(level, 1), n).astype(np.int32) for _ in range(3))
+            out = fl.DeviceArray(n, np.uint64)
+            fl.check(fl.lib().fl_hilbert_encode(dev(fl, x).p, dev(fl, y).p, dev(fl, z).p, None, level, out.p, n, None))
+            assert np.array_equal(out.to_host(), oracle.hilbert_encode(x, y, z, level))
+    n = 50000
+    lv = rng.integers(0, 20, n).astype(np.uint8)
+    x, y, z = (np.array([rng.integers(0, 1 << int(l)) for l in lv], np.int32) for _ in range(3))
+    out = fl.DeviceArray(n, np.uint64)
+    fl.check(fl.lib().fl_hilbert_key63(dev(fl, x).p, dev(fl, y).p, dev(fl, z).p, dev(fl, lv).p, 0, out.p, n, None))
+    k63 = oracle.key63(x, y, z, lv)
+    assert np.array_equal(out.to_host(), k63)
+    fl.check(fl.lib().fl_hilbert_key63(dev(fl, x).p, dev(fl, y).p, dev(fl, z).p, dev(fl, lv).p, 1, out.p, n, None))
+    assert np.array_equal(out.to_host(), k63 | lv.astype(np.uint64))
+
+
+def gpu_sort(fl, keys, vals, b, e):
+    n = len(keys)
+    k, v = dev(fl, keys), dev(fl, vals)
+    ka, va = fl.DeviceArray(max(n, 1), np.uint64), fl.DeviceArray(max(n, 1), np.uint32)
+    tb = fl.lib().fl_sort_temp_bytes(n)
+    t = fl.DeviceArray(max(tb, 1), np.uint8)
+    fl.check(fl.lib().fl_sort_pairs_u64u32(k.p, v.p, ka.p, va.p, n, b, e, t.p, tb, None))
+    return k.to_host(), v.to_host()
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 1000, 4096, 4097, 100003, (1 << 20) + 3])
+def test_sort_stable_matches_oracle(fl, oracle, n):
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, 1 << 63, n, dtype=np.uint64)
+    if n > 10:
+        keys[::5] = keys[1::5][: len(keys[::5])]
+        keys[3::17] &= np.uint64(0xFF00FF)
+    vals = np.arange(n, dtype=np.uint32)
+    for b, e in [(0, 64), (8, 40), (33, 63)]:
+        gk, gv = gpu_sort(fl, keys, vals, b, e)
+        ok, ov = oracle.sort_pairs(keys, vals, b, e)
+        assert np.array_equal(gk, ok) and np.array_equal(gv, ov)
+
+
+def test_sort_64m_key63_properties(fl):
+    """Full-size AMR micro-benchmark keys: 64M packed key63|level keys."""
+    n = 64 << 20
+    rng = np.random.default_rng(42)
+    keys = (rng.integers(0, 1 << 30, n, dtype=np.uint64) << np.uint64(33)) | rng.integers(0, 11, n).astype(np.uint64)
+    vals = np.arange(n, dtype=np.uint32)
+    gk, gv = gpu_sort(fl, keys, vals, 0, 64)
+    assert np.all(gk[1:] >= gk[:-1])
+    assert np.array_equal(keys[gv], gk)
+    same = gk[1:] == gk[:-1]
+    assert np.all(gv[1:][same] > gv[:-1][same])  # stability
+    assert np.array_equal(np.bincount(gv, minlength=n), np.ones(n))
+
+
+@pytest.mark.parametrize("n", [0, 1, 4095, 4096, 4097, 65537, 10_000_019])
+def test_scan_matches_oracle(fl, oracle, n):
+    rng = np.random.default_rng(n)
+    a = rng.integers(0, 7, n).astype(np.uint32)
+    d = dev(fl, a) if n else fl.DeviceArray(1, np.uint32)
+    out = fl.DeviceArray(max(n, 1), np.uint32)
+    tot = fl.DeviceArray(1, np.uint64)
+    tb = fl.lib().fl_scan_temp_bytes(n)
+    t = fl.DeviceArray(max(tb, 8), np.uint8)
+    fl.check(fl.lib().fl_scan_exclusive_u32(d.p, out.p, tot.p, n, t.p, tb, None))
+    ref, rt = oracle.scan_exclusive(a)
+    assert int(tot.to_host()[0]) == rt
+    if n:
+        assert np.array_equal(out.to_host()[:n], ref)
+
+
+@pytest.mark.parametrize("n", [1, 1000, 300001])
+def test_hash_matches_oracle(fl, oracle, n):
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, 1 << 50, n, dtype=np.uint64)
+    if n > 10:
+        keys[n // 2:n // 2 + n // 10] = keys[: n // 10]
+        keys[7] = np.uint64(0xFFFFFFFFFFFFFFFF)
+    vals = rng.permutation(n).astype(np.uint32)
+    cap = int(fl.lib().fl_hash_capacity(n))
+    assert cap == oracle.hash_capacity(n)
+    tk, tv = fl.DeviceArray(cap, np.uint64), fl.DeviceArray(cap, np.uint32)
+    pc = fl.DeviceArray(n, np.uint32)
+    fl.check(fl.lib().fl_hash_clear(tk.p, tv.p, cap, None))
+    fl.check(fl.lib().fl_hash_build(tk.p, tv.p, cap, dev(fl, keys).p, dev(fl, vals).p, n, pc.p, None))
+    q = np.concatenate([keys, rng.integers(1 << 51, 1 << 52, 5000, dtype=np.uint64)])
+    out = fl.DeviceArray(len(q), np.uint32)
+    fl.check(fl.lib().fl_hash_query(tk.p, tv.p, cap, dev(fl, q).p, out.p, len(q), None, None))
+    otk, otv, _ = oracle.hash_build(keys, vals, cap)
+    assert np.array_equal(out.to_host(), oracle.hash_query(otk, otv, cap, q))
+    assert pc.to_host().max() < 32
+
+
+def _gpu_cells(cs):
+    from fluidloom_amd import amr
+    return amr.GpuCells.from_host(x=cs.x, y=cs.y, z=cs.z, level=cs.level, state=cs.state, flag=cs.flag,
+                                  material=cs.material)
+
+
+def test_face_neighbors_match_oracle(fl, oracle):
+    from fluidloom_amd import amr
+    from tests.amr_util import random_octree
+    for seed, root in [(1, (2, 2, 2)), (5, (3, 2, 1))]:
+        c = random_octree(seed=seed, root=root, depth=4)
+        g = _gpu_cells(c)
+        nbr = amr.face_neighbors(g, root).to_host()[: 6 * c.n]
+        assert np.array_equal(nbr, oracle.face_neighbors(c, root))
+    x, y, z, lv = oracle.gen_cells(42, 200000, 10)
+    c = oracle.CellSet(len(x), x=x, y=y, z=z, level=lv)
+    g = _gpu_cells(c)
+    assert np.array_equal(amr.face_neighbors(g, (1, 1, 1)).to_host()[: 6 * c.n], oracle.face_neighbors(c, (1, 1, 1)))
+
+
+def _same_cells(gh, oc):
+    for nm in ("x", "y", "z", "level", "state", "material"):
+        assert np.array_equal(gh[nm], getattr(oc, nm)), nm
+
+
+def test_regrid_reference_mode_matches_oracle(fl, oracle):
+    """Micro-benchmark semantics at reduced size: random cells, levels 0-10,
+    split Bernoulli(0.05), merge -1 on complete sibling groups (0.03)."""
+    from fluidloom_amd import amr
+    from tests.c3_util import c3_cells
+    c = c3_cells(oracle, 300000, seed=42)
+    g = _gpu_cells(c)
+    out, st = amr.regrid(g, balance_mode=0, anchor_level=10, max_split_level=10, max_iters=64)
+    ref, res, gid, lead = oracle.regrid(c, 0, 10, 10, 64)
+    _same_cells(out.to_host(), ref)
+    assert st.num_groups == len(lead) and st.balance_iterations == res[0]
+    assert st.balance_converged == bool(res[1]) and st.balance_violations == res[2] and st.balance_marked == res[3]
+
+
+def test_regrid_lbm_mode_matches_oracle(fl, oracle):
+    from fluidloom_amd import amr
+    from tests.amr_util import random_octree
+    c = random_octree(seed=11, root=(2, 2, 2), depth=4, p_split=0.35)
+    rng = np.random.default_rng(3)
+    c.flag[:] = np.where(rng.random(c.n) < 0.1, 1, np.where(rng.random(c.n) < 0.5, -1, 0))
+    g = _gpu_cells(c)
+    out, st = amr.regrid(g, balance_mode=1, max_split_level=6, root=(2, 2, 2), merge_balance_check=1)
+    ref, res, gid, lead = oracle.regrid(c, 1, 19, 6, 64, (2, 2, 2), merge_balance_check=1)
+    _same_cells(out.to_host(), ref)
+    assert st.num_groups == len(lead)
+
+
+def test_balance_fixture_on_gpu(fl):
+    from fluidloom_amd import amr
+    g = amr.GpuCells.from_host(x=np.array([0, 4]), y=np.zeros(2), z=np.zeros(2), level=np.array([1, 3]))
+    out, st = amr.regrid(g, balance_mode=0, anchor_level=8, max_split_level=8, max_iters=10)
+    assert st.balance_converged and st.balance_violations > 0 and st.balance_marked > 0 and st.num_split == 1

@@ -1,0 +1,135 @@
+"""CPU: the FP64 lattice oracle pinned by closed forms (the reference has no
+LBM arithmetic to compare with; see the oracle module docstring)."""
+import numpy as np
+import pytest
+
+from tests.amr_util import box_region, refine_mesh, uniform_cells
+from tests.lbm_cases import OUT, PER, WALL, ZERO, case, oracle_solver, perturb
+
+
+def test_lattice_constants(oracle):
+    c, w, opp = oracle.lattice()
+    # the reference fixture's first 7 vectors and weights (test_parsing.cpp:23-30)
+    assert c[:7].tolist() == [[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]]
+    assert np.allclose(w[:7], [1 / 3] + [1 / 18] * 6)
+    assert abs(w.sum() - 1.0) < 1e-12  # LatticeDescriptor::validate (LatticeDescriptor.cpp:9-28)
+    assert np.allclose((w[:, None] * c).sum(0), 0)
+    assert np.allclose(np.einsum("q,qa,qb->ab", w, c, c), np.eye(3) / 3)
+    assert np.array_equal(c[opp], -c)
+    assert len({tuple(v) for v in c}) == 19
+
+
+def test_equilibrium_moments_and_fixed_point(oracle):
+    cs = case("periodic", oracle)
+    s = oracle_solver(oracle, cs)
+    c, w, _ = oracle.lattice()
+    f = s.get_f()
+    u = np.array(cs["u_init"])
+    assert np.allclose(f.sum(1), 1.0, atol=1e-14)
+    assert np.allclose(f @ c, u, atol=1e-14)
+    Pi = np.einsum("nq,qa,qb->nab", f, c, c)
+    assert np.allclose(Pi, np.eye(3) / 3 + np.outer(u, u), atol=1e-14)
+    s.step(5)
+    assert np.allclose(s.get_f(), f, atol=1e-14)
+
+
+def test_rest_state_stays_at_rest(oracle):
+    cs = case("cavity", oracle)
+    cs["bc_u"] = ZERO
+    s = oracle_solver(oracle, cs)
+    s.step(20)
+    _, w, _ = oracle.lattice()
+    assert np.abs(s.get_f() - w).max() < 1e-15
+
+
+def test_periodic_conservation(oracle):
+    cs = case("periodic", oracle)
+    s = oracle_solver(oracle, cs)
+    s.set_f(perturb(s.get_f(), 1, 1e-2))
+    m0, j0 = s.totals()
+    s.step(10)
+    m1, j1 = s.totals()
+    assert abs(m1 - m0) < 1e-12 and np.abs(j1 - j0).max() < 1e-13
+
+
+def test_taylor_green_decay(oracle):
+    N, tau, U = 32, 0.8, 0.01
+    c = uniform_cells(oracle, (N, N, 4))
+    s = oracle.Lbm(N, N, 4, 0, tau, [PER] * 6, ZERO, c.x, c.y, c.z, c.level, c.state)
+    k = 2 * np.pi / N
+    x, y = c.x + 0.5, c.y + 0.5
+    ux, uy = U * np.sin(k * x) * np.cos(k * y), -U * np.cos(k * x) * np.sin(k * y)
+    cv, w, _ = oracle.lattice()
+    uu = np.stack([ux, uy, np.zeros_like(ux)], 1)
+    cu = uu @ cv.T
+    feq = w * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * (uu ** 2).sum(1)[:, None])
+    s.set_f(feq)
+    T = 400
+    s.step(T)
+    _, u = s.moments()
+    nu = (tau - 0.5) / 3
+    amp = np.sqrt((u[:, 0] ** 2 + u[:, 1] ** 2).mean() / (ux ** 2 + uy ** 2).mean())
+    assert abs(amp / np.exp(-2 * nu * k * k * T) - 1) < 0.01
+
+
+def _couette(oracle, N, refined, steps):
+    U = 0.02
+    bcu = list(ZERO)
+    bcu[5] = (U, 0.0, 0.0)
+    base = uniform_cells(oracle, (4, 4, N))
+    if refined:
+        band = (0, 0, N // 2 - 1), (4, 4, N // 2 + 1)  # two coarse cells thick
+        c = refine_mesh(oracle, base, (4, 4, N), [(1, box_region(*band))], 1)
+        ml = 1
+    else:
+        c, ml = base, 0
+    s = oracle.Lbm(4, 4, N, ml, 1.0, [PER, PER, PER, PER, WALL, WALL], bcu, c.x, c.y, c.z, c.level, c.state)
+    s.step(steps)
+    _, u = s.moments()
+    zc = (s.fz + 0.5) / (2.0 ** s.flv)
+    exact = U * zc / N
+    assert np.abs(u[:, 1:]).max() < 1e-12
+    return np.abs(u[:, 0] - exact).max() / U
+
+
+def test_couette_profile_exact_on_uniform_grid(oracle):
+    # half-way bounce-back with the moving-wall term reproduces the linear
+    # profile to round-off
+    assert _couette(oracle, 8, False, 1500) < 1e-9
+
+
+def test_couette_profile_first_order_across_interfaces(oracle):
+    # the coarse->fine fill copies the parent's moments (zeroth order in
+    # space), so the interface error is O(dx): halving dx halves it
+    e8 = _couette(oracle, 8, True, 1500)
+    e16 = _couette(oracle, 16, True, 12000)
+    assert e8 < 0.03 and e16 < 0.6 * e8
+
+
+def test_uniform_flow_through_coarse_fine_interfaces(oracle):
+    cs = case("patch2", oracle)
+    s = oracle_solver(oracle, cs)
+    s.step(6)
+    rho, u = s.moments()
+    assert np.abs(rho - 1).max() < 1e-13
+    assert np.abs(u - np.array(cs["u_init"])).max() < 1e-13
+
+
+def test_refined_mass_drift_small(oracle):
+    cs = case("patch2", oracle)
+    s = oracle_solver(oracle, cs)
+    s.set_f(perturb(s.get_f(), 3, 1e-3))
+    m0, _ = s.totals()
+    s.step(10)
+    m1, _ = s.totals()
+    assert abs(m1 - m0) / m0 < 1e-4
+
+
+def test_sphere_case_runs_and_drags(oracle):
+    cs = case("sphere3", oracle)
+    s = oracle_solver(oracle, cs)
+    s.step(3)
+    F = s.force()
+    rho, u = s.moments()
+    assert np.isfinite(F).all() and F[0] > 0
+    assert np.isfinite(u).all() and np.abs(rho - 1).max() < 0.1
