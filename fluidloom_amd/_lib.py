@@ -92,6 +92,8 @@ _SIGS = {
     "fl_mem_info": (C.c_int, [C.POINTER(sz), C.POINTER(sz)]),
     "fl_hilbert_encode": (C.c_int, [vp, vp, vp, vp, C.c_int, vp, sz, vp]),
     "fl_hilbert_decode": (C.c_int, [vp, vp, C.c_int, vp, vp, vp, sz, vp]),
+    "fl_hilbert_encode_aos4": (C.c_int, [vp, C.c_int, vp, sz, vp]),
+    "fl_hilbert_decode_aos4": (C.c_int, [vp, C.c_int, vp, sz, vp]),
     "fl_hilbert_key63": (C.c_int, [vp, vp, vp, vp, C.c_int, vp, sz, vp]),
     "fl_cell_hkey": (C.c_int, [vp, vp, vp, vp, vp, sz, vp]),
     "fl_rotdir_encode": (C.c_int, [vp, vp, vp, C.c_int, vp, sz, vp]),
@@ -114,6 +116,8 @@ _SIGS = {
                            C.POINTER(u64), vp]),
     "fl_merge": (C.c_int, [C.POINTER(Cells), sz, vp, vp, C.POINTER(Cells), sz, vp, sz,
                            C.POINTER(u64), vp]),
+    "fl_gen_cells": (C.c_int, [C.POINTER(Cells), sz, u64, C.c_int, vp]),
+    "fl_gen_flags": (C.c_int, [C.POINTER(Cells), sz, u64, C.c_double, C.c_double, vp, sz, vp]),
     "fl_mark_ball": (C.c_int, [C.POINTER(Cells), sz, C.c_double, C.c_double, C.c_double, C.c_double,
                                C.c_int, C.POINTER(u64), vp]),
     "fl_mark_solid_ball": (C.c_int, [C.POINTER(Cells), sz, C.c_double, C.c_double, C.c_double,
@@ -129,6 +133,8 @@ _SIGS = {
     "fl_lbm_get_f": (C.c_int, [vp, vp, vp, vp, vp, sz, vp]),
     "fl_lbm_set_f": (C.c_int, [vp, vp, vp, vp, vp, sz, vp]),
     "fl_lbm_moments": (C.c_int, [vp, vp, vp, vp, vp, sz, vp, vp]),
+    "fl_lbm_vorticity": (C.c_int, [vp, vp, vp, vp, vp, sz, vp]),
+    "fl_lbm_regrid": (C.c_int, [vp, vp, vp, vp, vp, vp, sz]),
     "fl_lbm_totals": (C.c_int, [vp, vp, vp]),
     "fl_lbm_force": (C.c_int, [vp, vp]),
     "fl_lbm_stream": (C.c_int, [vp, C.POINTER(vp)]),

@@ -1,0 +1,105 @@
+"""AMR integer micro-benchmark on the device: 64M seeded leaf cells at levels
+0-10 in a 2^10 root domain, split Bernoulli(0.05), merge Bernoulli(0.03) on
+complete sibling groups; Hilbert keys, 64-bit key sort, one regrid pass
+(reference balance rule to fixpoint, split, merge, scan compaction) and the
+hash face-neighbour rebuild.  Each stage is timed with HIP events; GB/s is
+computed from the algorithmic bytes of that stage.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import amr
+from ._lib import DeviceArray, RegridConfig, RegridResult, check, lib
+
+
+class _Timer:
+    def __init__(self):
+        self.a, self.b = C.c_void_p(), C.c_void_p()
+        check(lib().fl_event_create(C.byref(self.a)))
+        check(lib().fl_event_create(C.byref(self.b)))
+
+    def __enter__(self):
+        check(lib().fl_device_sync())
+        check(lib().fl_event_record(self.a, None))
+        return self
+
+    def __exit__(self, *exc):
+        check(lib().fl_event_record(self.b, None))
+        ms = C.c_float(0)
+        check(lib().fl_event_elapsed_ms(self.a, self.b, C.byref(ms)))
+        self.ms = ms.value
+
+
+def run(n=64 << 20, seed=42, p_split=0.05, p_merge=0.03, max_level=10, log=None):
+    L = lib()
+    cells = amr.GpuCells(n)
+    cells.n = n
+    cs = cells.struct()
+    out = {"cells": n}
+    t = _Timer()
+    with t:
+        check(L.fl_gen_cells(C.byref(cs), n, seed, max_level, None))
+    tb = L.fl_regrid_temp_bytes(n)
+    tmp = DeviceArray(tb, np.uint8)
+    check(L.fl_gen_flags(C.byref(cs), n, seed, p_split, p_merge, tmp.p, tb, None))
+    # keygen: key63 | level (21 B/cell: 12 coords + 1 level + 8 key)
+    keys = DeviceArray(n, np.uint64)
+    with t:
+        check(L.fl_hilbert_key63(cells.x.p, cells.y.p, cells.z.p, cells.level.p, 1, keys.p, n, None))
+    out["keygen_ms"] = t.ms
+    out["keygen_gbs"] = 21 * n / t.ms / 1e6
+    # 64-bit key sort with values = original index (skips constant digits)
+    vals = DeviceArray(n, np.uint32)
+    check(L.fl_iota_u32(vals.p, n, None))
+    ka, va = DeviceArray(n, np.uint64), DeviceArray(n, np.uint32)
+    stb = L.fl_sort_temp_bytes(n)
+    st = DeviceArray(stb, np.uint8)
+    with t:
+        check(L.fl_sort_pairs_u64u32(keys.p, vals.p, ka.p, va.p, n, 0, 64, st.p, stb, None))
+    out["sort_ms"] = t.ms
+    out["sort_gkeys_s"] = n / t.ms / 1e6
+    del ka, va, st
+    # regrid pass
+    cap = n + n // 2
+    new = amr.GpuCells(cap)
+    cfg = RegridConfig()
+    cfg.balance_mode, cfg.anchor_level, cfg.max_split_level, cfg.max_balance_iters = 0, max_level, max_level, 64
+    cfg.root[0] = cfg.root[1] = cfg.root[2] = 1
+    res = RegridResult()
+    ns = new.struct()
+    w0 = time.perf_counter()
+    with t:
+        check(L.fl_regrid(C.byref(cfg), C.byref(cs), n, C.byref(ns), cap, None, None, tmp.p, tb, C.byref(res),
+                          None))
+    out["regrid_ms"] = t.ms
+    out["regrid_wall_ms"] = (time.perf_counter() - w0) * 1e3
+    out.update(cells_out=int(res.num_cells_out), split=int(res.num_split), merged_groups=int(res.num_groups),
+               balance_iterations=int(res.balance_iterations), balance_converged=bool(res.balance_converged),
+               balance_marked=int(res.balance_marked))
+    new.n = int(res.num_cells_out)
+    cells.free()
+    del tmp
+    # hash face-neighbour rebuild over the new cells
+    m = new.n
+    hcap = int(L.fl_hash_capacity(m))
+    hk, tk, tv = DeviceArray(m, np.uint64), DeviceArray(hcap, np.uint64), DeviceArray(hcap, np.uint32)
+    nbr = DeviceArray(6 * m, np.uint32)
+    root = (C.c_int32 * 3)(1, 1, 1)
+    nsn = new.struct()
+    with t:
+        check(L.fl_face_neighbors(C.byref(nsn), m, root, hk.p, tk.p, tv.p, hcap, nbr.p, None))
+    out["neighbors_ms"] = t.ms
+    # hash build alone on the sorted keys (24 B/key)
+    with t:
+        check(L.fl_hash_clear(tk.p, tv.p, hcap, None))
+        check(L.fl_hash_build(tk.p, tv.p, hcap, keys.p, vals.p, min(n, m), None, None))
+    out["hash_build_ms"] = t.ms
+    out["hash_build_gkeys_s"] = min(n, m) / t.ms / 1e6
+    new.free()
+    if log:
+        log(f"amr micro-benchmark: {out}")
+    return out

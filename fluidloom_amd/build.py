@@ -41,12 +41,32 @@ def sources() -> list[Path]:
     return sorted(CSRC.glob("*.hip"))
 
 
+
 def _compile(src: Path) -> Path:
     out = OBJ / (src.stem + ".o")
     cmd = [HIPCC, *COMMON, "-c", str(src), "-o", str(out)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{r.stderr}")
+    return out
+
+
+def build_cpp_tests(verbose: bool = True) -> Path:
+    """The C++ API test program (tests/cpp/test_api.cpp), linked against the
+    library and the oracle (its checker)."""
+    src = ROOT / "tests" / "cpp" / "test_api.cpp"
+    out_dir = ROOT / "tests" / "cpp" / "_build"
+    out_dir.mkdir(exist_ok=True)
+    out = out_dir / "test_api"
+    oracle_dir = ROOT / "oracle" / "_build"
+    cmd = [HIPCC, "-std=c++17", "-O2", f"-I{ROOT / 'include'}", str(src), "-o", str(out),
+           f"-L{LIBDIR}", "-lfluidloom_hip", f"-Wl,-rpath,{LIBDIR}", f"-L{oracle_dir}", "-loracle",
+           f"-Wl,-rpath,{oracle_dir}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ API tests failed to build:\n{r.stderr}")
+    if verbose:
+        print(f"built {out}", flush=True)
     return out
 
 

@@ -10,6 +10,7 @@
 // parent order) are fixed by scans: groups follow leader index order,
 // children follow parent order, compaction is stable.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -470,6 +471,61 @@ __global__ __launch_bounds__(kBlock) void k_mark_solid(CellsW c, size_t n, doubl
     if (local) atomicAdd(count, local);
 }
 
+// ---------------------------------------------------------------- synthetic inputs
+// splitmix64 (Steele, Lea & Flood 2014 finaliser constants)
+__host__ __device__ inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// level ~ U{0..max_level}, coords ~ U[0, 2^level): four independent
+// splitmix64 streams keyed by (seed, index).
+__global__ __launch_bounds__(kBlock) void k_gen_cells(uint64_t seed, size_t n, int max_level, CellsW c) {
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const uint64_t b = seed * 0x100000001B3ull + (uint64_t)i * 4;
+        const uint64_t r0 = splitmix64(b), r1 = splitmix64(b + 1), r2 = splitmix64(b + 2), r3 = splitmix64(b + 3);
+        const int L = (int)((r0 >> 32) % (uint64_t)(max_level + 1));
+        const uint32_t m = (1u << L) - 1u;
+        c.level[i] = (uint8_t)L;
+        c.x[i] = (int32_t)((uint32_t)(r1 >> 32) & m);
+        c.y[i] = (int32_t)((uint32_t)(r2 >> 32) & m);
+        c.z[i] = (int32_t)((uint32_t)(r3 >> 32) & m);
+        c.state[i] = 0;
+        c.flag[i] = 0;
+        c.material[i] = 0;
+    }
+}
+
+// split flag: Bernoulli(p_split) from stream (seed ^ 0x5A17, i)
+__global__ __launch_bounds__(kBlock) void k_flag_split(uint64_t seed, size_t n, uint64_t thresh, CellsW c) {
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+        c.flag[i] = splitmix64((seed ^ 0x5A17ull) * 0x100000001B3ull + i) < thresh ? 1 : 0;
+}
+
+// merge flags: a leader (smallest-index holder of an all-even position,
+// level > 0) whose 8 siblings all exist is chosen with Bernoulli(p_merge) from
+// stream (seed ^ 0x3E26, leader index); its siblings still at flag 0 get -1.
+__global__ __launch_bounds__(kBlock) void k_flag_merge(uint64_t seed, size_t n, uint64_t thresh, CellHash h, CellsW c) {
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const int32_t x = c.x[i], y = c.y[i], z = c.z[i];
+        const int L = c.level[i];
+        if (L == 0 || (x & 1) || (y & 1) || (z & 1)) continue;
+        if (find_cell(h, L, x, y, z) != (uint32_t)i) continue;
+        if (splitmix64((seed ^ 0x3E26ull) * 0x100000001B3ull + i) >= thresh) continue;
+        uint32_t sib[8];
+        bool ok = true;
+        for (int q = 0; q < 8 && ok; ++q) {
+            sib[q] = find_cell(h, L, x | (q & 1), y | ((q >> 1) & 1), z | ((q >> 2) & 1));
+            ok = sib[q] != kInvalid;
+        }
+        if (!ok) continue;
+        for (int q = 0; q < 8; ++q)
+            if (c.flag[sib[q]] == 0) c.flag[sib[q]] = -1;
+    }
+}
+
 // ---------------------------------------------------------------- workspace
 struct RegridLayout {
     size_t keys, tk, tv, shadow, viol, gid, leaders, counters, ws0, ws1, ws2, total;
@@ -625,6 +681,35 @@ fl_status fl_mark_solid_ball(fl_cells* c, size_t n, double cx, double cy, double
     FL_HIP(hipFreeAsync(cnt, st));
     FL_HIP(hipStreamSynchronize(st));
     *h_marked = h;
+    return FL_OK;
+}
+
+fl_status fl_gen_cells(fl_cells* c, size_t n, uint64_t seed, int max_level, fl_stream s) {
+    FL_REQUIRE(c && c->x && c->y && c->z && c->level && c->state && c->flag && c->material, "null pointer");
+    FL_REQUIRE(max_level >= 0 && max_level <= 19, "max_level must be in 0..19");
+    if (n) hipLaunchKernelGGL(k_gen_cells, dim3(sgrid(n)), dim3(kBlock), 0, FL_STREAM(s), seed, n, max_level, wview(c));
+    FL_CHECK_LAUNCH();
+    return FL_OK;
+}
+
+fl_status fl_gen_flags(fl_cells* c, size_t n, uint64_t seed, double p_split, double p_merge, void* d_temp,
+                       size_t temp_bytes, fl_stream s) {
+    FL_REQUIRE(c && c->x && c->flag, "null pointer");
+    FL_REQUIRE(p_split >= 0 && p_split <= 1 && p_merge >= 0 && p_merge <= 1, "probabilities must be in [0,1]");
+    RegridLayout L = regrid_layout(n > 0 ? n : 1);
+    FL_REQUIRE(temp_bytes >= L.total, "temp buffer too small (see fl_regrid_temp_bytes)");
+    if (n == 0) return FL_OK;
+    hipStream_t st = FL_STREAM(s);
+    auto th = [](double p) { return p >= 1.0 ? ~0ull : (uint64_t)std::ldexp(p, 64); };
+    hipLaunchKernelGGL(k_flag_split, dim3(sgrid(n)), dim3(kBlock), 0, st, seed, n, th(p_split), wview(c));
+    FL_CHECK_LAUNCH();
+    char* b = static_cast<char*>(d_temp);
+    fl_status r = build_cell_hash(cview(c), n, reinterpret_cast<uint64_t*>(b + L.keys), reinterpret_cast<uint64_t*>(b + L.tk),
+                                  reinterpret_cast<uint32_t*>(b + L.tv), L.cap, st);
+    if (r) return r;
+    CellHash h{reinterpret_cast<uint64_t*>(b + L.tk), reinterpret_cast<uint32_t*>(b + L.tv), L.cap};
+    hipLaunchKernelGGL(k_flag_merge, dim3(sgrid(n)), dim3(kBlock), 0, st, seed, n, th(p_merge), h, wview(c));
+    FL_CHECK_LAUNCH();
     return FL_OK;
 }
 

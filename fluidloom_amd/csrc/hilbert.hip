@@ -21,7 +21,8 @@ __global__ __launch_bounds__(kBlock) void k_encode(const int32_t* __restrict__ x
                                                    const int32_t* __restrict__ y,
                                                    const int32_t* __restrict__ z,
                                                    const uint8_t* __restrict__ lev, int lev_scalar,
-                                                   int mode, uint64_t* __restrict__ out, size_t n) {
+                                                   int mode, uint64_t* __restrict__ out, size_t n,
+                                                   size_t stride) {
     // mode 0: raw encode; 1: key63; 2: key63 | level
     __shared__ HilbertLds L;
     hilbert_lds_init(L, false);
@@ -32,7 +33,7 @@ __global__ __launch_bounds__(kBlock) void k_encode(const int32_t* __restrict__ x
         if (l > 21) {
             k = kEmptyKey;
         } else {
-            k = hilbert_encode_lds(L, (uint32_t)x[i], (uint32_t)y[i], (uint32_t)z[i], l);
+            k = hilbert_encode_lds(L, (uint32_t)x[i * stride], (uint32_t)y[i * stride], (uint32_t)z[i * stride], l);
             if (mode == 1) k <<= 3 * (21 - l);
             else if (mode == 2) k = (k << (3 * (21 - l))) | (uint64_t)l;
         }
@@ -43,7 +44,7 @@ __global__ __launch_bounds__(kBlock) void k_encode(const int32_t* __restrict__ x
 __global__ __launch_bounds__(kBlock) void k_decode(const uint64_t* __restrict__ key,
                                                    const uint8_t* __restrict__ lev, int lev_scalar,
                                                    int32_t* __restrict__ x, int32_t* __restrict__ y,
-                                                   int32_t* __restrict__ z, size_t n) {
+                                                   int32_t* __restrict__ z, size_t n, size_t stride) {
     __shared__ HilbertLds L;
     hilbert_lds_init(L, true);
     __syncthreads();
@@ -51,7 +52,7 @@ __global__ __launch_bounds__(kBlock) void k_decode(const uint64_t* __restrict__ 
         int l = lev ? (int)lev[i] : lev_scalar;
         int32_t a = 0, b = 0, c = 0;
         if (l > 0 && l <= 21) hilbert_decode_lds(L, key[i], l, &a, &b, &c);
-        x[i] = a; y[i] = b; z[i] = c;
+        x[i * stride] = a; y[i * stride] = b; z[i * stride] = c;
     }
 }
 
@@ -107,7 +108,7 @@ fl_status fl_hilbert_encode(const int32_t* d_x, const int32_t* d_y, const int32_
     FL_REQUIRE(d_x && d_y && d_z && d_key, "null pointer");
     FL_REQUIRE(d_level || (level_scalar >= 0 && level_scalar <= 21), "level must be in 0..21");
     hipLaunchKernelGGL(k_encode, dim3(stride_grid(n)), dim3(kBlock), 0, FL_STREAM(s), d_x, d_y, d_z,
-                       d_level, level_scalar, 0, d_key, n);
+                       d_level, level_scalar, 0, d_key, n, (size_t)1);
     FL_CHECK_LAUNCH();
     return FL_OK;
 }
@@ -118,7 +119,7 @@ fl_status fl_hilbert_decode(const uint64_t* d_key, const uint8_t* d_level, int l
     FL_REQUIRE(d_x && d_y && d_z && d_key, "null pointer");
     FL_REQUIRE(d_level || (level_scalar >= 0 && level_scalar <= 21), "level must be in 0..21");
     hipLaunchKernelGGL(k_decode, dim3(stride_grid(n)), dim3(kBlock), 0, FL_STREAM(s), d_key, d_level,
-                       level_scalar, d_x, d_y, d_z, n);
+                       level_scalar, d_x, d_y, d_z, n, (size_t)1);
     FL_CHECK_LAUNCH();
     return FL_OK;
 }
@@ -129,7 +130,27 @@ fl_status fl_hilbert_key63(const int32_t* d_x, const int32_t* d_y, const int32_t
     if (n == 0) return FL_OK;
     FL_REQUIRE(d_x && d_y && d_z && d_key && d_level, "null pointer");
     hipLaunchKernelGGL(k_encode, dim3(stride_grid(n)), dim3(kBlock), 0, FL_STREAM(s), d_x, d_y, d_z,
-                       d_level, 0, pack_level ? 2 : 1, d_key, n);
+                       d_level, 0, pack_level ? 2 : 1, d_key, n, (size_t)1);
+    FL_CHECK_LAUNCH();
+    return FL_OK;
+}
+
+fl_status fl_hilbert_encode_aos4(const int32_t* d_xyzw, int level, uint64_t* d_key, size_t n, fl_stream s) {
+    if (n == 0) return FL_OK;
+    FL_REQUIRE(d_xyzw && d_key, "null pointer");
+    FL_REQUIRE(level >= 0 && level <= 21, "level must be in 0..21");
+    hipLaunchKernelGGL(k_encode, dim3(stride_grid(n)), dim3(kBlock), 0, FL_STREAM(s), d_xyzw, d_xyzw + 1,
+                       d_xyzw + 2, (const uint8_t*)nullptr, level, 0, d_key, n, (size_t)4);
+    FL_CHECK_LAUNCH();
+    return FL_OK;
+}
+
+fl_status fl_hilbert_decode_aos4(const uint64_t* d_key, int level, int32_t* d_xyzw, size_t n, fl_stream s) {
+    if (n == 0) return FL_OK;
+    FL_REQUIRE(d_xyzw && d_key, "null pointer");
+    FL_REQUIRE(level >= 0 && level <= 21, "level must be in 0..21");
+    hipLaunchKernelGGL(k_decode, dim3(stride_grid(n)), dim3(kBlock), 0, FL_STREAM(s), d_key,
+                       (const uint8_t*)nullptr, level, d_xyzw, d_xyzw + 1, d_xyzw + 2, n, (size_t)4);
     FL_CHECK_LAUNCH();
     return FL_OK;
 }

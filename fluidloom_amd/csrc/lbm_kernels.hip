@@ -273,6 +273,197 @@ __global__ __launch_bounds__(kWG) void k_inject(T* __restrict__ A, T* __restrict
     }
 }
 
+// |vorticity| (level-0 units) at listed fluid-leaf slots: central
+// differences over same-level fluid-leaf neighbours, one-sided at the edge
+// of the level, zero when neither side is a fluid leaf.
+template <typename T>
+__device__ inline bool leaf_velocity(const LevelDev& L, const T* __restrict__ A, int64_t s, double* u) {
+    if (s < 0) return false;
+    const bool simple = (L.bflags[s >> 6] & 1u) != 0;
+    if (!simple && !(L.cflags[s] & kActiveBit)) return false;
+    double r = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        double f = (double)Store<T>::load(A, (size_t)i * L.S + (size_t)s, i);
+        r += f;
+        jx += cx(i) * f;
+        jy += cy(i) * f;
+        jz += cz(i) * f;
+    }
+    u[0] = jx / r;
+    u[1] = jy / r;
+    u[2] = jz / r;
+    return true;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_vorticity(LevelDev L, const T* __restrict__ A, const int64_t* __restrict__ slots,
+                                                   size_t n, double scale, double* __restrict__ out) {
+    for (size_t k = (size_t)blockIdx.x * kWG + threadIdx.x; k < n; k += (size_t)gridDim.x * kWG) {
+        const int64_t s = slots[k];
+        double u0[3];
+        if (!leaf_velocity<T>(L, A, s, u0)) {
+            out[k] = 0.0;
+            continue;
+        }
+        const int b = (int)(s >> 6), l = (int)(s & 63);
+        const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
+        double g[3][3];
+        for (int a = 0; a < 3; ++a) {
+            const int dx = a == 0, dy = a == 1, dz = a == 2;
+            double up[3], um[3];
+            const bool hp = leaf_velocity<T>(L, A, slot_of(L, b, lx + dx, ly + dy, lz + dz), up);
+            const bool hm = leaf_velocity<T>(L, A, slot_of(L, b, lx - dx, ly - dy, lz - dz), um);
+            for (int c = 0; c < 3; ++c)
+                g[a][c] = hp && hm ? 0.5 * (up[c] - um[c]) : (hp ? up[c] - u0[c] : (hm ? u0[c] - um[c] : 0.0));
+        }
+        const double wx = g[1][2] - g[2][1], wy = g[2][0] - g[0][2], wz = g[0][1] - g[1][0];
+        out[k] = sqrt(wx * wx + wy * wy + wz * wz) * scale;
+    }
+}
+
+// Population transfer of a regrid: copy / split child / merged parent / rest.
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_remap_copy(const T* __restrict__ Ao, size_t So, const uint32_t* __restrict__ dst,
+                                                    const uint32_t* __restrict__ src, uint32_t n, T* __restrict__ A,
+                                                    T* __restrict__ B, size_t S) {
+    for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < n; k += gridDim.x * kWG) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const T v = Ao[(size_t)i * So + src[k]];
+            A[(size_t)i * S + dst[k]] = v;
+            B[(size_t)i * S + dst[k]] = v;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_remap_split(LevelDev Lc, BCDev bc, const T* __restrict__ Acprev,
+                                                     const uint32_t* __restrict__ dst, const uint32_t* __restrict__ src,
+                                                     uint32_t n, typename Store<T>::R fac, T* __restrict__ A,
+                                                     T* __restrict__ B, size_t S) {
+    using R = typename Store<T>::R;
+    for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < n; k += gridDim.x * kWG) {
+        R f[Q];
+        gather_slot<T, R>(Lc, bc, Acprev, src[k], f);
+        R rho, ux, uy, uz;
+        moments(f, rho, ux, uy, uz);
+        const R usq = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const R e = feq(i, rho, ux, uy, uz, usq);
+            const R v = e + fac * (f[i] - e);
+            Store<T>::store(A, (size_t)i * S + dst[k], i, v);
+            Store<T>::store(B, (size_t)i * S + dst[k], i, v);
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_remap_merge(LevelDev Lf, BCDev bc, const T* __restrict__ Afprev,
+                                                     const uint32_t* __restrict__ dst, const uint32_t* __restrict__ src8,
+                                                     uint32_t n, typename Store<T>::R fac, T* __restrict__ A,
+                                                     T* __restrict__ B, size_t S) {
+    using R = typename Store<T>::R;
+    for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < n; k += gridDim.x * kWG) {
+        R sr = 0, sjx = 0, sjy = 0, sjz = 0, sneq[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) sneq[i] = 0;
+        for (int q = 0; q < 8; ++q) {
+            R f[Q];
+            gather_slot<T, R>(Lf, bc, Afprev, src8[(size_t)8 * k + q], f);
+            R rho, ux, uy, uz;
+            moments(f, rho, ux, uy, uz);
+            const R usq = ux * ux + uy * uy + uz * uz;
+            sr += rho;
+            sjx += rho * ux;
+            sjy += rho * uy;
+            sjz += rho * uz;
+#pragma unroll
+            for (int i = 0; i < Q; ++i) sneq[i] += f[i] - feq(i, rho, ux, uy, uz, usq);
+        }
+        const R r = sr * (R)0.125;
+        const R ux = (sjx * (R)0.125) / r, uy = (sjy * (R)0.125) / r, uz = (sjz * (R)0.125) / r;
+        const R usq = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const R v = feq(i, r, ux, uy, uz, usq) + fac * (sneq[i] * (R)0.125);
+            Store<T>::store(A, (size_t)i * S + dst[k], i, v);
+            Store<T>::store(B, (size_t)i * S + dst[k], i, v);
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_remap_rest(const uint32_t* __restrict__ dst, uint32_t n, T* __restrict__ A,
+                                                    T* __restrict__ B, size_t S) {
+    using R = typename Store<T>::R;
+    for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < n; k += gridDim.x * kWG) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const R v = (R)wq(i);
+            Store<T>::store(A, (size_t)i * S + dst[k], i, v);
+            Store<T>::store(B, (size_t)i * S + dst[k], i, v);
+        }
+    }
+}
+
+// Ghost slots after a regrid: equilibrium of the covering coarse leaf (c2f)
+// or of the 8 children's average density and momentum (f2c), both buffers.
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_ghost_eq_c2f(const uint32_t* __restrict__ ghost, const uint32_t* __restrict__ capidx,
+                                                      const uint32_t* __restrict__ cap_slots, uint32_t n,
+                                                      const T* __restrict__ Ac, size_t Sc, T* __restrict__ A,
+                                                      T* __restrict__ B, size_t S) {
+    using R = typename Store<T>::R;
+    for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < n; k += gridDim.x * kWG) {
+        const size_t cs = cap_slots[capidx[k]];
+        R f[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f[i] = Store<T>::load(Ac, (size_t)i * Sc + cs, i);
+        R rho, ux, uy, uz;
+        moments(f, rho, ux, uy, uz);
+        const R usq = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const R e = feq(i, rho, ux, uy, uz, usq);
+            Store<T>::store(A, (size_t)i * S + ghost[k], i, e);
+            Store<T>::store(B, (size_t)i * S + ghost[k], i, e);
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_ghost_eq_f2c(const uint32_t* __restrict__ ghost, const uint32_t* __restrict__ child8,
+                                                      uint32_t n, const T* __restrict__ Af, size_t Sf, T* __restrict__ A,
+                                                      T* __restrict__ B, size_t S) {
+    using R = typename Store<T>::R;
+    for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < n; k += gridDim.x * kWG) {
+        R sr = 0, sjx = 0, sjy = 0, sjz = 0;
+        for (int q = 0; q < 8; ++q) {
+            const size_t cs = child8[(size_t)8 * k + q];
+            R f[Q];
+#pragma unroll
+            for (int i = 0; i < Q; ++i) f[i] = Store<T>::load(Af, (size_t)i * Sf + cs, i);
+            R rho, ux, uy, uz;
+            moments(f, rho, ux, uy, uz);
+            sr += rho;
+            sjx += rho * ux;
+            sjy += rho * uy;
+            sjz += rho * uz;
+        }
+        const R r = sr * (R)0.125;
+        const R ux = (sjx * (R)0.125) / r, uy = (sjy * (R)0.125) / r, uz = (sjz * (R)0.125) / r;
+        const R usq = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const R e = feq(i, r, ux, uy, uz, usq);
+            Store<T>::store(A, (size_t)i * S + ghost[k], i, e);
+            Store<T>::store(B, (size_t)i * S + ghost[k], i, e);
+        }
+    }
+}
+
 unsigned lgrid(size_t n) {
     size_t g = (n + kWG - 1) / kWG;
     if (g > 4096) g = 4096;
@@ -356,6 +547,43 @@ hipError_t launch_inject(T* A, T* B, size_t S, const int64_t* slots, size_t n, c
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_vorticity(const LevelDev& L, const T* A, const int64_t* slots, size_t n, double scale, double* out,
+                            hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_vorticity<T>, dim3(lgrid(n)), dim3(kWG), 0, s, L, A, slots, n, scale, out);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_remap(const RemapArgs<T>& r, hipStream_t s) {
+    if (r.ncopy)
+        hipLaunchKernelGGL(k_remap_copy<T>, dim3(lgrid(r.ncopy)), dim3(kWG), 0, s, r.copy_src_buf, r.So, r.copy_dst,
+                           r.copy_src, r.ncopy, r.A, r.B, r.S);
+    if (r.nsplit)
+        hipLaunchKernelGGL(k_remap_split<T>, dim3(lgrid(r.nsplit)), dim3(kWG), 0, s, r.Lc, r.bc, r.split_prev,
+                           r.split_dst, r.split_src, r.nsplit, (typename Store<T>::R)r.split_fac, r.A, r.B, r.S);
+    if (r.nmerge)
+        hipLaunchKernelGGL(k_remap_merge<T>, dim3(lgrid(r.nmerge)), dim3(kWG), 0, s, r.Lf, r.bc, r.merge_prev,
+                           r.merge_dst, r.merge_src8, r.nmerge, (typename Store<T>::R)r.merge_fac, r.A, r.B, r.S);
+    if (r.nrest)
+        hipLaunchKernelGGL(k_remap_rest<T>, dim3(lgrid(r.nrest)), dim3(kWG), 0, s, r.rest_dst, r.nrest, r.A, r.B, r.S);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_ghost_eq(const uint32_t* c2f_ghost, const uint32_t* c2f_cap, const uint32_t* cap_slots, uint32_t nc2f,
+                           const T* Ac, size_t Sc, const uint32_t* f2c_ghost, const uint32_t* f2c_child, uint32_t nf2c,
+                           const T* Af, size_t Sf, T* A, T* B, size_t S, hipStream_t s) {
+    if (nc2f)
+        hipLaunchKernelGGL(k_ghost_eq_c2f<T>, dim3(lgrid(nc2f)), dim3(kWG), 0, s, c2f_ghost, c2f_cap, cap_slots, nc2f,
+                           Ac, Sc, A, B, S);
+    if (nf2c)
+        hipLaunchKernelGGL(k_ghost_eq_f2c<T>, dim3(lgrid(nf2c)), dim3(kWG), 0, s, f2c_ghost, f2c_child, nf2c, Af, Sf,
+                           A, B, S);
+    return hipGetLastError();
+}
+
 #define FL_INST(T)                                                                                           \
     template hipError_t launch_collide<T>(const LevelDev&, const BCDev&, const T*, T*, double, hipStream_t); \
     template hipError_t launch_capture<T>(const LevelDev&, const BCDev&, const T*, const uint32_t*, uint32_t, \
@@ -369,7 +597,13 @@ hipError_t launch_inject(T* A, T* B, size_t S, const int64_t* slots, size_t n, c
     template hipError_t launch_force<T>(const uint32_t*, uint32_t, size_t, const T*, double, double*, unsigned, \
                                         hipStream_t);                                                        \
     template hipError_t launch_extract<T>(const T*, size_t, const int64_t*, size_t, double*, hipStream_t);   \
-    template hipError_t launch_inject<T>(T*, T*, size_t, const int64_t*, size_t, const double*, hipStream_t);
+    template hipError_t launch_inject<T>(T*, T*, size_t, const int64_t*, size_t, const double*, hipStream_t); \
+    template hipError_t launch_vorticity<T>(const LevelDev&, const T*, const int64_t*, size_t, double, double*,  \
+                                            hipStream_t);                                                    \
+    template hipError_t launch_remap<T>(const RemapArgs<T>&, hipStream_t);                                   \
+    template hipError_t launch_ghost_eq<T>(const uint32_t*, const uint32_t*, const uint32_t*, uint32_t, const T*, \
+                                           size_t, const uint32_t*, const uint32_t*, uint32_t, const T*, size_t, \
+                                           T*, T*, size_t, hipStream_t);
 
 FL_INST(float)
 FL_INST(double)

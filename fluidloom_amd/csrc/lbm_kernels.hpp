@@ -34,6 +34,43 @@ template <typename T>
 hipError_t launch_extract(const T* A, size_t S, const int64_t* slots, size_t n, double* out, hipStream_t s);
 template <typename T>
 hipError_t launch_inject(T* A, T* B, size_t S, const int64_t* slots, size_t n, const double* in, hipStream_t s);
+template <typename T>
+hipError_t launch_vorticity(const LevelDev& L, const T* A, const int64_t* slots, size_t n, double scale, double* out,
+                            hipStream_t s);
+
+// One level's population transfer after a regrid.
+template <typename T>
+struct RemapArgs {
+    T* A = nullptr;
+    T* B = nullptr;
+    size_t S = 0;
+    const T* copy_src_buf = nullptr;  // old level, current buffer
+    size_t So = 0;
+    const uint32_t* copy_dst = nullptr;
+    const uint32_t* copy_src = nullptr;
+    uint32_t ncopy = 0;
+    LevelDev Lc{};                     // old level l-1
+    const T* split_prev = nullptr;     // its previous buffer
+    const uint32_t* split_dst = nullptr;
+    const uint32_t* split_src = nullptr;
+    uint32_t nsplit = 0;
+    double split_fac = 0;
+    LevelDev Lf{};                     // old level l+1
+    const T* merge_prev = nullptr;
+    const uint32_t* merge_dst = nullptr;
+    const uint32_t* merge_src8 = nullptr;
+    uint32_t nmerge = 0;
+    double merge_fac = 0;
+    const uint32_t* rest_dst = nullptr;
+    uint32_t nrest = 0;
+    BCDev bc{};
+};
+template <typename T>
+hipError_t launch_remap(const RemapArgs<T>& r, hipStream_t s);
+template <typename T>
+hipError_t launch_ghost_eq(const uint32_t* c2f_ghost, const uint32_t* c2f_cap, const uint32_t* cap_slots, uint32_t nc2f,
+                           const T* Ac, size_t Sc, const uint32_t* f2c_ghost, const uint32_t* f2c_child, uint32_t nf2c,
+                           const T* Af, size_t Sf, T* A, T* B, size_t S, hipStream_t s);
 
 }  // namespace lbm
 }  // namespace fl

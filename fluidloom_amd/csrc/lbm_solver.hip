@@ -699,19 +699,9 @@ fl_status force(fl_lbm_solver* S, double* F) {
     return FL_OK;
 }
 
-}  // namespace
-}  // namespace lbm
-}  // namespace fl
-
-using namespace fl;
-using namespace fl::lbm;
-
-extern "C" {
-
-fl_status fl_lbm_create(const fl_lbm_params* p, const int32_t* h_x, const int32_t* h_y, const int32_t* h_z,
-                        const uint8_t* h_level, const uint8_t* h_state, size_t n, fl_stream s,
-                        fl_lbm_solver** out) {
-    FL_REQUIRE(p && out, "null pointer");
+fl_status create_solver(const fl_lbm_params* p, const int32_t* h_x, const int32_t* h_y, const int32_t* h_z,
+                        const uint8_t* h_level, const uint8_t* h_state, size_t n, hipStream_t stream,
+                        std::unique_ptr<fl_lbm_solver>& out) {
     FL_REQUIRE(n == 0 || (h_x && h_y && h_z && h_level && h_state), "null leaf arrays");
     FL_REQUIRE(p->max_level >= 0 && p->max_level < 20, "max_level must be in 0..19");
     FL_REQUIRE(p->nx0 > 0 && p->ny0 > 0 && p->nz0 > 0 && p->nx0 % 4 == 0 && p->ny0 % 4 == 0 && p->nz0 % 4 == 0,
@@ -722,11 +712,10 @@ fl_status fl_lbm_create(const fl_lbm_params* p, const int32_t* h_x, const int32_
     for (int a = 0; a < 3; ++a)
         FL_REQUIRE((p->bc[2 * a] == FL_BC_PERIODIC) == (p->bc[2 * a + 1] == FL_BC_PERIODIC),
                    "periodic faces must come in pairs");
-    *out = nullptr;
     std::unique_ptr<fl_lbm_solver> S(new fl_lbm_solver());
     S->p = *p;
     S->L = p->max_level + 1;
-    S->stream = FL_STREAM(s);
+    S->stream = stream;
     for (int f = 0; f < 6; ++f) {
         S->bc.type[f] = p->bc[f];
         for (int a = 0; a < 3; ++a) {
@@ -742,6 +731,165 @@ fl_status fl_lbm_create(const fl_lbm_params* p, const int32_t* h_x, const int32_
         case FL_STORE_F16: r = upload_and_init<_Float16>(S.get()); break;
         default: r = upload_and_init<double>(S.get()); break;
     }
+    if (r) return r;
+    out = std::move(S);
+    return FL_OK;
+}
+
+template <typename T>
+fl_status vorticity(fl_lbm_solver* S, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv,
+                    size_t n, double* out) {
+    std::vector<int64_t> slots;
+    query_slots(S, x, y, z, lv, n, slots);
+    for (size_t k = 0; k < n; ++k) out[k] = 0.0;
+    std::vector<double> tmp;
+    for (int l = 0; l < S->L; ++l) {
+        std::vector<int64_t> ls;
+        std::vector<size_t> idx;
+        for (size_t k = 0; k < n; ++k)
+            if (lv[k] == l) { ls.push_back(slots[k]); idx.push_back(k); }
+        if (ls.empty()) continue;
+        fl_status r = ensure_query(S, ls.size());
+        if (r) return r;
+        FL_HIP(hipMemcpyAsync(S->qslots, ls.data(), ls.size() * 8, hipMemcpyHostToDevice, S->stream));
+        DevLevel& D = S->d[l];
+        FL_HIP(launch_vorticity<T>(D.ld, static_cast<const T*>(D.buf[D.cur]), S->qslots, ls.size(),
+                                   std::ldexp(1.0, l), S->qbuf, S->stream));
+        tmp.resize(ls.size());
+        FL_HIP(hipMemcpyAsync(tmp.data(), S->qbuf, tmp.size() * 8, hipMemcpyDeviceToHost, S->stream));
+        FL_HIP(hipStreamSynchronize(S->stream));
+        for (size_t j = 0; j < ls.size(); ++j) out[idx[j]] = tmp[j];
+    }
+    return FL_OK;
+}
+
+// Population transfer from the old solver O into the freshly built N.
+template <typename T>
+fl_status remap(fl_lbm_solver* O, fl_lbm_solver* N) {
+    std::vector<void*> tmp_allocs;
+    auto up = [&](const std::vector<uint32_t>& v) -> const uint32_t* {
+        if (v.empty()) return nullptr;
+        void* d = nullptr;
+        if (hipMalloc(&d, v.size() * 4) != hipSuccess) return nullptr;
+        tmp_allocs.push_back(d);
+        (void)hipMemcpyAsync(d, v.data(), v.size() * 4, hipMemcpyHostToDevice, N->stream);
+        return static_cast<const uint32_t*>(d);
+    };
+    fl_status st = FL_OK;
+    for (int l = 0; l < N->L && st == FL_OK; ++l) {
+        const HostLevel& H = N->h[l];
+        std::vector<uint32_t> cdst, csrc, sdst, ssrc, mdst, msrc, rdst;
+        for (uint32_t b = 0; b < H.nb; ++b)
+            for (int loc = 0; loc < 64; ++loc) {
+                const size_t s = (size_t)b * 64 + loc;
+                if (H.stype[s] != T_FLUID) continue;
+                const int32_t x = H.bcoord[3 * b] * 4 + (loc & 3), y = H.bcoord[3 * b + 1] * 4 + ((loc >> 2) & 3),
+                              z = H.bcoord[3 * b + 2] * 4 + (loc >> 4);
+                const HostLevel& OH = O->h[l];
+                int64_t os = OH.slot_at(x, y, z);
+                if (os >= 0 && OH.stype[(size_t)os] == T_FLUID) {
+                    cdst.push_back((uint32_t)s);
+                    csrc.push_back((uint32_t)os);
+                    continue;
+                }
+                if (l > 0) {
+                    const HostLevel& OC = O->h[l - 1];
+                    int64_t ps = OC.slot_at(x >> 1, y >> 1, z >> 1);
+                    if (ps >= 0 && OC.stype[(size_t)ps] == T_FLUID) {
+                        sdst.push_back((uint32_t)s);
+                        ssrc.push_back((uint32_t)ps);
+                        continue;
+                    }
+                }
+                if (l + 1 < O->L) {
+                    const HostLevel& OF = O->h[l + 1];
+                    uint32_t ch[8];
+                    bool all = true;
+                    for (int q = 0; q < 8 && all; ++q) {
+                        int64_t c = OF.slot_at(2 * x + (q & 1), 2 * y + ((q >> 1) & 1), 2 * z + ((q >> 2) & 1));
+                        all = c >= 0 && OF.stype[(size_t)c] == T_FLUID;
+                        if (all) ch[q] = (uint32_t)c;
+                    }
+                    if (all) {
+                        mdst.push_back((uint32_t)s);
+                        msrc.insert(msrc.end(), ch, ch + 8);
+                        continue;
+                    }
+                }
+                rdst.push_back((uint32_t)s);
+            }
+        RemapArgs<T> a;
+        DevLevel& D = N->d[l];
+        a.A = static_cast<T*>(D.buf[0]);
+        a.B = static_cast<T*>(D.buf[1]);
+        a.S = D.ld.S;
+        a.bc = N->bc;
+        const DevLevel& OD = O->d[l];
+        a.copy_src_buf = static_cast<const T*>(OD.buf[OD.cur]);
+        a.So = OD.ld.S;
+        a.copy_dst = up(cdst);
+        a.copy_src = up(csrc);
+        a.ncopy = (uint32_t)cdst.size();
+        if (l > 0) {
+            const DevLevel& OC = O->d[l - 1];
+            a.Lc = OC.ld;
+            a.split_prev = static_cast<const T*>(OC.buf[1 - OC.cur]);
+            a.split_fac = (1.0 - 1.0 / D.tau) * (D.tau / (2.0 * OC.tau));
+        }
+        a.split_dst = up(sdst);
+        a.split_src = up(ssrc);
+        a.nsplit = (uint32_t)sdst.size();
+        if (l + 1 < O->L) {
+            const DevLevel& OF = O->d[l + 1];
+            a.Lf = OF.ld;
+            a.merge_prev = static_cast<const T*>(OF.buf[1 - OF.cur]);
+            a.merge_fac = (1.0 - 1.0 / D.tau) * (2.0 * D.tau / OF.tau);
+        }
+        a.merge_dst = up(mdst);
+        a.merge_src8 = up(msrc);
+        a.nmerge = (uint32_t)mdst.size();
+        a.rest_dst = up(rdst);
+        a.nrest = (uint32_t)rdst.size();
+        hipError_t e = launch_remap<T>(a, N->stream);
+        if (e != hipSuccess) st = fl::hip_fail(e, "launch_remap", __FILE__, __LINE__);
+        D.cur = 0;
+    }
+    for (int l = 0; l < N->L && st == FL_OK; ++l) {
+        const HostLevel& H = N->h[l];
+        DevLevel& D = N->d[l];
+        const T* Ac = l > 0 ? static_cast<const T*>(N->d[l - 1].buf[0]) : nullptr;
+        const size_t Sc = l > 0 ? N->d[l - 1].ld.S : 0;
+        const uint32_t* capsl = l > 0 ? N->d[l - 1].cap_slots : nullptr;
+        const T* Af = l + 1 < N->L ? static_cast<const T*>(N->d[l + 1].buf[0]) : nullptr;
+        const size_t Sf = l + 1 < N->L ? N->d[l + 1].ld.S : 0;
+        hipError_t e = launch_ghost_eq<T>(D.c2f_ghost, D.c2f_cap, capsl, (uint32_t)H.c2f_ghost.size(), Ac, Sc,
+                                          D.f2c_ghost, D.f2c_child, (uint32_t)H.f2c_ghost.size(), Af, Sf,
+                                          static_cast<T*>(D.buf[0]), static_cast<T*>(D.buf[1]), D.ld.S, N->stream);
+        if (e != hipSuccess) st = fl::hip_fail(e, "launch_ghost_eq", __FILE__, __LINE__);
+    }
+    hipError_t e = hipStreamSynchronize(N->stream);
+    for (void* p : tmp_allocs) (void)hipFree(p);
+    if (st) return st;
+    if (e != hipSuccess) return fl::hip_fail(e, "remap", __FILE__, __LINE__);
+    return FL_OK;
+}
+
+}  // namespace
+}  // namespace lbm
+}  // namespace fl
+
+using namespace fl;
+using namespace fl::lbm;
+
+extern "C" {
+
+fl_status fl_lbm_create(const fl_lbm_params* p, const int32_t* h_x, const int32_t* h_y, const int32_t* h_z,
+                        const uint8_t* h_level, const uint8_t* h_state, size_t n, fl_stream s,
+                        fl_lbm_solver** out) {
+    FL_REQUIRE(p && out, "null pointer");
+    *out = nullptr;
+    std::unique_ptr<fl_lbm_solver> S;
+    fl_status r = create_solver(p, h_x, h_y, h_z, h_level, h_state, n, FL_STREAM(s), S);
     if (r) return r;
     *out = S.release();
     return FL_OK;
@@ -875,6 +1023,38 @@ fl_status fl_lbm_time_collide(fl_lbm_solver* S, int level, int iters, float* ms_
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     *ms_per_launch = ms / iters;
+    return FL_OK;
+}
+
+fl_status fl_lbm_vorticity(fl_lbm_solver* S, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv,
+                           size_t n, double* out) {
+    FL_REQUIRE(S && (n == 0 || (x && y && z && lv && out)), "null pointer");
+    switch (S->p.storage) {
+        case FL_STORE_F32: return vorticity<float>(S, x, y, z, lv, n, out);
+        case FL_STORE_F16: return vorticity<_Float16>(S, x, y, z, lv, n, out);
+        default: return vorticity<double>(S, x, y, z, lv, n, out);
+    }
+}
+
+fl_status fl_lbm_regrid(fl_lbm_solver* S, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv,
+                        const uint8_t* state, size_t n) {
+    FL_REQUIRE(S, "null solver");
+    std::unique_ptr<fl_lbm_solver> N;
+    fl_lbm_params p = S->p;
+    fl_status r = create_solver(&p, x, y, z, lv, state, n, S->stream, N);
+    if (r) return r;
+    switch (S->p.storage) {
+        case FL_STORE_F32: r = remap<float>(S, N.get()); break;
+        case FL_STORE_F16: r = remap<_Float16>(S, N.get()); break;
+        default: r = remap<double>(S, N.get()); break;
+    }
+    if (r) return r;
+    // the solver handle keeps its identity; its contents become the new mesh
+    std::swap(S->h, N->h);
+    std::swap(S->d, N->d);
+    std::swap(S->allocs, N->allocs);
+    std::swap(S->partial, N->partial);
+    std::swap(S->device_bytes, N->device_bytes);
     return FL_OK;
 }
 

@@ -93,6 +93,23 @@ class LbmSolver:
         check(lib().fl_lbm_force(self.h, F.ctypes.data))
         return F
 
+    def vorticity(self):
+        out = np.empty(len(self.fx))
+        check(lib().fl_lbm_vorticity(self.h, self.fx.ctypes.data, self.fy.ctypes.data, self.fz.ctypes.data,
+                                     self.flv.ctypes.data, len(self.fx), out.ctypes.data))
+        return out
+
+    def regrid(self, x, y, z, level, state):
+        """Switch to a new leaf list, carrying the populations over."""
+        x, y, z = (np.ascontiguousarray(a, np.int32) for a in (x, y, z))
+        level = np.ascontiguousarray(level, np.uint8)
+        state = np.ascontiguousarray(state, np.uint8)
+        check(lib().fl_lbm_regrid(self.h, x.ctypes.data, y.ctypes.data, z.ctypes.data, level.ctypes.data,
+                                  state.ctypes.data, len(x)))
+        self.x, self.y, self.z, self.level, self.state = x, y, z, level, state
+        fl = state == 0
+        self.fx, self.fy, self.fz, self.flv = (np.ascontiguousarray(a) for a in (x[fl], y[fl], z[fl], level[fl]))
+
     def time_collide(self, level, iters):
         ms = C.c_float(0)
         check(lib().fl_lbm_time_collide(self.h, int(level), int(iters), C.byref(ms)))

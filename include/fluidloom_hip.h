@@ -71,6 +71,12 @@ fl_status fl_hilbert_encode(const int32_t* d_x, const int32_t* d_y, const int32_
 /* replaces hilbert::decode (HilbertCodec.cpp:82-115), kernels/hilbert.cl:118-130 */
 fl_status fl_hilbert_decode(const uint64_t* d_key, const uint8_t* d_level, int level_scalar,
                             int32_t* d_x, int32_t* d_y, int32_t* d_z, size_t n, fl_stream s);
+/* The same codec over int4-strided coordinates (the cl_int3 / 16-byte layout
+ * of test_hilbert_encode / test_hilbert_decode, kernels/hilbert.cl:104-130). */
+fl_status fl_hilbert_encode_aos4(const int32_t* d_xyzw, int level, uint64_t* d_key, size_t n,
+                                 fl_stream s);
+fl_status fl_hilbert_decode_aos4(const uint64_t* d_key, int level, int32_t* d_xyzw, size_t n,
+                                 fl_stream s);
 /* prefix-aligned 63-bit key encode(x,y,z,L) << 3*(21-L); with pack_level != 0
  * the level is or-ed into the low 5 bits (requires every level <= 19) so that
  * one sort orders by (key, level). */
@@ -189,6 +195,16 @@ fl_status fl_merge(const fl_cells* c, size_t n, uint32_t* d_group_id, uint32_t* 
                    fl_cells* parents, size_t parent_capacity, void* d_temp, size_t temp_bytes,
                    uint64_t* h_num_groups, fl_stream s);
 
+/* Seeded synthetic inputs of the AMR integer micro-benchmark (splitmix64
+ * streams keyed by (seed, index); identical to the oracle's generator):
+ * fl_gen_cells: level ~ U{0..max_level}, coords ~ U[0, 2^level), state 0,
+ * flag 0.  fl_gen_flags: split flag 1 with probability p_split; each complete
+ * sibling group whose leader draws below p_merge gets -1 on its members that
+ * are still at 0 (temp: fl_regrid_temp_bytes(n)). */
+fl_status fl_gen_cells(fl_cells* c, size_t n, uint64_t seed, int max_level, fl_stream s);
+fl_status fl_gen_flags(fl_cells* c, size_t n, uint64_t seed, double p_split, double p_merge,
+                       void* d_temp, size_t temp_bytes, fl_stream s);
+
 /* Refinement criteria used by the benchmark meshes (synchronous):
  * fl_mark_ball sets flag = 1 on FLUID cells with level < target_level whose
  * cube (level-0 units) is within `radius` of the centre; fl_mark_solid_ball
@@ -258,6 +274,19 @@ fl_status fl_lbm_set_f(fl_lbm_solver* solver, const int32_t* h_x, const int32_t*
 fl_status fl_lbm_moments(fl_lbm_solver* solver, const int32_t* h_x, const int32_t* h_y,
                          const int32_t* h_z, const uint8_t* h_level, size_t n, double* h_rho,
                          double* h_u);
+/* |vorticity| in level-0 units at the given leaves (regrid criterion):
+ * central differences over same-level fluid leaves, one-sided at level
+ * edges (synchronous). */
+fl_status fl_lbm_vorticity(fl_lbm_solver* solver, const int32_t* h_x, const int32_t* h_y,
+                           const int32_t* h_z, const uint8_t* h_level, size_t n, double* h_out);
+/* Replace the mesh by a new leaf list (from fl_regrid) and carry the
+ * populations over: unchanged leaves are copied, children of split cells
+ * get the parent's state with f_neq rescaled, merged parents the 8-child
+ * average -- the AdaptMeshNode step of the reference
+ * (src/runtime/AdaptMeshNode.cpp:9-64), whose GPU path drops the fields
+ * (AdaptationEngine.cpp:540-549).  Rebuilds the solver's device layout. */
+fl_status fl_lbm_regrid(fl_lbm_solver* solver, const int32_t* h_x, const int32_t* h_y,
+                        const int32_t* h_z, const uint8_t* h_level, const uint8_t* h_state, size_t n);
 /* FP64 device reductions (synchronous) */
 fl_status fl_lbm_totals(fl_lbm_solver* solver, double* h_mass, double* h_mom3);
 fl_status fl_lbm_force(fl_lbm_solver* solver, double* h_force3);

@@ -624,3 +624,35 @@ void or_gen_cells(uint64_t seed, size_t n, int max_level, int32_t* x, int32_t* y
         z[i] = (int32_t)((uint32_t)(r3 >> 32) & m);
     }
 }
+
+/* Flags of the AMR micro-benchmark, identical to fl_gen_flags: split with
+ * probability p_split (stream seed ^ 0x5A17), complete sibling groups whose
+ * leader draws below p_merge (stream seed ^ 0x3E26) get -1 on members still
+ * at 0.  Thresholds are p * 2^64 compared with the raw 64-bit draw. */
+void or_gen_flags(const or_cells* c, uint64_t seed, uint64_t split_thresh, uint64_t merge_thresh) {
+    size_t n = c->n;
+    for (size_t i = 0; i < n; ++i)
+        c->flag[i] = or_splitmix64((seed ^ 0x5A17ull) * 0x100000001B3ull + i) < split_thresh ? 1 : 0;
+    cellmap m;
+    uint64_t* keys = (uint64_t*)malloc((n ? n : 1) * sizeof(uint64_t));
+    cell_keys(c, keys);
+    cm_build(&m, c, keys);
+    for (size_t i = 0; i < n; ++i) {
+        int32_t x = c->x[i], y = c->y[i], z = c->z[i];
+        int L = c->level[i];
+        if (L == 0 || (x & 1) || (y & 1) || (z & 1)) continue;
+        if (cm_cell(&m, L, x, y, z) != (uint32_t)i) continue;
+        if (or_splitmix64((seed ^ 0x3E26ull) * 0x100000001B3ull + i) >= merge_thresh) continue;
+        uint32_t sib[8];
+        int ok = 1;
+        for (int q = 0; q < 8 && ok; ++q) {
+            sib[q] = cm_cell(&m, L, x | (q & 1), y | ((q >> 1) & 1), z | ((q >> 2) & 1));
+            ok = sib[q] != OR_INVALID;
+        }
+        if (!ok) continue;
+        for (int q = 0; q < 8; ++q)
+            if (c->flag[sib[q]] == 0) c->flag[sib[q]] = -1;
+    }
+    cm_free(&m);
+    free(keys);
+}

@@ -104,6 +104,8 @@ uint64_t or_splitmix64(uint64_t x);
 void or_gen_cells(uint64_t seed, size_t n, int max_level, int32_t* x, int32_t* y, int32_t* z,
                   uint8_t* level);
 
+void or_gen_flags(const or_cells* c, uint64_t seed, uint64_t split_thresh, uint64_t merge_thresh);
+
 #ifdef __cplusplus
 }
 #endif

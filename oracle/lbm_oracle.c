@@ -468,3 +468,155 @@ void or_lbm_plant_coupling_error(or_lbm* s, double c2f_mult, double f2c_mult) {
     s->c2f_mult = c2f_mult;
     s->f2c_mult = f2c_mult;
 }
+
+/* ---- regrid support ------------------------------------------------------
+ * Vorticity magnitude (level-0 units) at fluid leaves: central differences
+ * over same-level fluid-leaf neighbours along each axis, one-sided when only
+ * one side is a fluid leaf, zero when neither is; |w| * 2^level. */
+static int leaf_u(const or_lbm* s, int l, int x, int y, int z, double* u) {
+    const level_t* v = &s->lv[l];
+    int p[3] = {x, y, z};
+    int e[3] = {v->X, v->Y, v->Z};
+    for (int a = 0; a < 3; ++a) {
+        if (p[a] >= 0 && p[a] < e[a]) continue;
+        if (s->p.bc[2 * a] != OR_BC_PERIODIC) return 0;
+        p[a] = (p[a] + e[a]) % e[a];
+    }
+    int64_t q = pidx(v, p[0], p[1], p[2]);
+    if (v->type[q] != T_FLUID) return 0;
+    double f[Q], r;
+    for (int i = 0; i < Q; ++i) f[i] = v->f[v->cur][(size_t)i * v->N + q];
+    moments(f, &r, &u[0], &u[1], &u[2]);
+    return 1;
+}
+
+void or_lbm_vorticity(const or_lbm* s, const int32_t* x, const int32_t* y, const int32_t* z,
+                      const uint8_t* level, size_t n, double* out) {
+    for (size_t k = 0; k < n; ++k) {
+        int l = level[k];
+        double u0[3], g[3][3];
+        if (l >= s->L || !leaf_u(s, l, x[k], y[k], z[k], u0)) { out[k] = 0; continue; }
+        for (int a = 0; a < 3; ++a) {
+            int dp[3] = {0, 0, 0};
+            dp[a] = 1;
+            double up[3], um[3];
+            int hp = leaf_u(s, l, x[k] + dp[0], y[k] + dp[1], z[k] + dp[2], up);
+            int hm = leaf_u(s, l, x[k] - dp[0], y[k] - dp[1], z[k] - dp[2], um);
+            for (int b = 0; b < 3; ++b) {
+                if (hp && hm) g[a][b] = 0.5 * (up[b] - um[b]);
+                else if (hp) g[a][b] = up[b] - u0[b];
+                else if (hm) g[a][b] = u0[b] - um[b];
+                else g[a][b] = 0.0;
+            }
+        }
+        double wx = g[1][2] - g[2][1], wy = g[2][0] - g[0][2], wz = g[0][1] - g[1][0];
+        out[k] = sqrt(wx * wx + wy * wy + wz * wz) * ldexp(1.0, l);
+    }
+}
+
+/* New solver over a new leaf list with the populations carried over:
+ * a leaf present before keeps its post-collision populations; a child of a
+ * split parent gets feq(parent) + (1-1/tau_f)(tau_f/(2 tau_c)) fneq(parent);
+ * a merged parent gets feq(avg) + (1-1/tau_c)(2 tau_c/tau_f) avg fneq of its
+ * 8 children (pre-collision states gathered from the previous buffer, as in
+ * the ghost fill); anything else starts at equilibrium(1, 0).  Both buffers
+ * of the new solver hold the result; ghost slots restart at the
+ * equilibrium of the leaves covering them. */
+or_lbm* or_lbm_regrid(const or_lbm* o, const int32_t* x, const int32_t* y, const int32_t* z,
+                      const uint8_t* level, const uint8_t* state, size_t n, int* err) {
+    const double zero[3] = {0, 0, 0};
+    or_lbm* s = or_lbm_create(&o->p, x, y, z, level, state, n, 1.0, zero, err);
+    if (!s) return NULL;
+    for (int l = 0; l < s->L; ++l) {
+        level_t* v = &s->lv[l];
+        for (size_t k = 0; k < v->nfluid; ++k) {
+            int64_t p = v->fluid[k];
+            int px, py, pz;
+            pxyz(v, p, &px, &py, &pz);
+            double f[Q];
+            const level_t* ov = &o->lv[l];
+            if (ov->type[p] == T_FLUID) {
+                for (int i = 0; i < Q; ++i) f[i] = ov->f[ov->cur][(size_t)i * ov->N + p];
+            } else if (l > 0 && o->lv[l - 1].type[pidx(&o->lv[l - 1], px >> 1, py >> 1, pz >> 1)] == T_FLUID) {
+                const level_t* oc = &o->lv[l - 1];
+                double fp[Q], r, ux, uy, uz;
+                gather(o, oc, pidx(oc, px >> 1, py >> 1, pz >> 1), oc->f[1 - oc->cur], fp);
+                moments(fp, &r, &ux, &uy, &uz);
+                double fac = (1.0 - 1.0 / v->tau) * (v->tau / (2.0 * oc->tau));
+                for (int i = 0; i < Q; ++i) {
+                    double e = feq(r, ux, uy, uz, i);
+                    f[i] = e + fac * (fp[i] - e);
+                }
+            } else {
+                int all = l + 1 < o->L;
+                for (int q = 0; q < 8 && all; ++q) {
+                    const level_t* of = &o->lv[l + 1];
+                    all = of->type[pidx(of, 2 * px + (q & 1), 2 * py + ((q >> 1) & 1), 2 * pz + ((q >> 2) & 1))] == T_FLUID;
+                }
+                if (all) {
+                    const level_t* of = &o->lv[l + 1];
+                    double sr = 0, sjx = 0, sjy = 0, sjz = 0, sneq[Q];
+                    for (int i = 0; i < Q; ++i) sneq[i] = 0;
+                    for (int q = 0; q < 8; ++q) {
+                        double fk[Q], r, ux, uy, uz;
+                        gather(o, of, pidx(of, 2 * px + (q & 1), 2 * py + ((q >> 1) & 1), 2 * pz + ((q >> 2) & 1)),
+                               of->f[1 - of->cur], fk);
+                        moments(fk, &r, &ux, &uy, &uz);
+                        sr += r; sjx += r * ux; sjy += r * uy; sjz += r * uz;
+                        for (int i = 0; i < Q; ++i) sneq[i] += fk[i] - feq(r, ux, uy, uz, i);
+                    }
+                    double r = sr * 0.125;
+                    double ux = (sjx * 0.125) / r, uy = (sjy * 0.125) / r, uz = (sjz * 0.125) / r;
+                    double fac = (1.0 - 1.0 / v->tau) * (2.0 * v->tau / of->tau);
+                    for (int i = 0; i < Q; ++i) f[i] = feq(r, ux, uy, uz, i) + fac * (sneq[i] * 0.125);
+                } else {
+                    for (int i = 0; i < Q; ++i) f[i] = feq(1.0, 0, 0, 0, i);
+                }
+            }
+            for (int i = 0; i < Q; ++i) {
+                v->f[0][(size_t)i * v->N + p] = f[i];
+                v->f[1][(size_t)i * v->N + p] = f[i];
+            }
+        }
+    }
+    /* ghosts restart at the equilibrium of what covers them: the coarse
+     * leaf (c2f) or the average density and momentum of the 8 children
+     * (f2c), in both buffers */
+    for (int l = 0; l < s->L; ++l) {
+        level_t* v = &s->lv[l];
+        for (size_t k = 0; k < v->nc2f; ++k) {
+            int64_t g = v->c2f[k];
+            const level_t* c = &s->lv[l - 1];
+            double f[Q], r, ux, uy, uz;
+            for (int i = 0; i < Q; ++i) f[i] = c->f[0][(size_t)i * c->N + v->c2f_src[g]];
+            moments(f, &r, &ux, &uy, &uz);
+            for (int i = 0; i < Q; ++i) {
+                double e = feq(r, ux, uy, uz, i);
+                v->f[0][(size_t)i * v->N + g] = e;
+                v->f[1][(size_t)i * v->N + g] = e;
+            }
+        }
+        for (size_t k = 0; k < v->nf2c; ++k) {
+            int64_t g = v->f2c[k];
+            const level_t* fv = &s->lv[l + 1];
+            int px, py, pz;
+            pxyz(v, g, &px, &py, &pz);
+            double sr = 0, sjx = 0, sjy = 0, sjz = 0;
+            for (int q = 0; q < 8; ++q) {
+                int64_t cp = pidx(fv, 2 * px + (q & 1), 2 * py + ((q >> 1) & 1), 2 * pz + ((q >> 2) & 1));
+                double f[Q], r, ux, uy, uz;
+                for (int i = 0; i < Q; ++i) f[i] = fv->f[0][(size_t)i * fv->N + cp];
+                moments(f, &r, &ux, &uy, &uz);
+                sr += r; sjx += r * ux; sjy += r * uy; sjz += r * uz;
+            }
+            double r = sr * 0.125;
+            double ux = (sjx * 0.125) / r, uy = (sjy * 0.125) / r, uz = (sjz * 0.125) / r;
+            for (int i = 0; i < Q; ++i) {
+                double e = feq(r, ux, uy, uz, i);
+                v->f[0][(size_t)i * v->N + g] = e;
+                v->f[1][(size_t)i * v->N + g] = e;
+            }
+        }
+    }
+    return s;
+}

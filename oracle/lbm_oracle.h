@@ -58,6 +58,14 @@ void or_lbm_totals(const or_lbm* s, double* mass, double mom[3]);
 /* momentum-exchange force on solid leaves, in level-0 lattice units */
 void or_lbm_force(const or_lbm* s, double F[3]);
 
+/* Vorticity magnitude at the given leaves, level-0 units (regrid criterion) */
+void or_lbm_vorticity(const or_lbm* s, const int32_t* x, const int32_t* y, const int32_t* z,
+                      const uint8_t* level, size_t n, double* out);
+/* A new solver on a new leaf list with populations carried over from `o`
+ * (copy / split / merge rules in lbm_oracle.c). */
+or_lbm* or_lbm_regrid(const or_lbm* o, const int32_t* x, const int32_t* y, const int32_t* z,
+                      const uint8_t* level, const uint8_t* state, size_t n, int* err);
+
 /* Tests only: multiply the c2f / f2c f_neq rescale factors (1.0 = the
  * scheme) so that a test can plant a plausible mistake and see it fail. */
 void or_lbm_plant_coupling_error(or_lbm* s, double c2f_mult, double f2c_mult);

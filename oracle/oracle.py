@@ -87,6 +87,7 @@ def lib():
         L.or_compact_append.restype = sz
         L.or_compact_append.argtypes = [C.c_void_p, vp, C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_gen_cells.argtypes = [C.c_uint64, sz, C.c_int, vp, vp, vp, vp]
+        L.or_gen_flags.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64]
         L.or_lbm_create.restype = vp
         L.or_lbm_create.argtypes = [C.c_void_p, vp, vp, vp, vp, vp, sz, C.c_double, vp,
                                     C.POINTER(C.c_int)]
@@ -99,6 +100,9 @@ def lib():
         L.or_lbm_force.argtypes = [vp, vp]
         L.or_lbm_lattice.argtypes = [vp, vp, vp]
         L.or_lbm_plant_coupling_error.argtypes = [vp, C.c_double, C.c_double]
+        L.or_lbm_vorticity.argtypes = [vp, vp, vp, vp, vp, sz, vp]
+        L.or_lbm_regrid.restype = vp
+        L.or_lbm_regrid.argtypes = [vp, vp, vp, vp, vp, vp, sz, C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -306,6 +310,17 @@ def regrid(c: CellSet, balance_mode, anchor_level, max_split_level, max_iters=64
     return compact_append(c, gid, ch, par), res, gid, lead
 
 
+def _thresh(p):
+    return (1 << 64) - 1 if p >= 1.0 else int(np.ldexp(p, 64))
+
+
+def gen_flags(c: CellSet, seed, p_split, p_merge):
+    """In place: the micro-benchmark's split / merge flags (as fl_gen_flags)."""
+    s = c.struct()
+    lib().or_gen_flags(C.byref(s), seed, _thresh(p_split), _thresh(p_merge))
+    return c
+
+
 def gen_cells(seed, n, max_level):
     x, y, z = (np.empty(n, np.int32) for _ in range(3))
     lv = np.empty(n, np.uint8)
@@ -386,6 +401,27 @@ class Lbm:
         F = np.zeros(3)
         lib().or_lbm_force(self.h, _p(F))
         return F
+
+    def vorticity(self):
+        out = np.empty(len(self.fx))
+        lib().or_lbm_vorticity(self.h, _p(self.fx), _p(self.fy), _p(self.fz), _p(self.flv), len(self.fx),
+                               _p(out))
+        return out
+
+    def regrid(self, x, y, z, level, state):
+        """Replace the mesh by a new leaf list, carrying the populations over."""
+        x, y, z = (np.ascontiguousarray(a, np.int32) for a in (x, y, z))
+        level = np.ascontiguousarray(level, np.uint8)
+        state = np.ascontiguousarray(state, np.uint8)
+        err = C.c_int(0)
+        h = lib().or_lbm_regrid(self.h, _p(x), _p(y), _p(z), _p(level), _p(state), len(x), C.byref(err))
+        if not h:
+            raise ValueError(f"oracle mesh error {err.value}")
+        lib().or_lbm_destroy(self.h)
+        self.h = h
+        self.x, self.y, self.z, self.level, self.state = x, y, z, level, state
+        fl = state == 0
+        self.fx, self.fy, self.fz, self.flv = x[fl], y[fl], z[fl], level[fl]
 
     def __del__(self):
         try:

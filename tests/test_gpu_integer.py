@@ -183,3 +183,31 @@ def test_balance_fixture_on_gpu(fl):
     g = amr.GpuCells.from_host(x=np.array([0, 4]), y=np.zeros(2), z=np.zeros(2), level=np.array([1, 3]))
     out, st = amr.regrid(g, balance_mode=0, anchor_level=8, max_split_level=8, max_iters=10)
     assert st.balance_converged and st.balance_violations > 0 and st.balance_marked > 0 and st.num_split == 1
+
+
+def test_synthetic_generators_match_oracle(fl, oracle):
+    from fluidloom_amd import amr
+    from tests.c3_util import c3_cells
+    n = 300000
+    g = amr.GpuCells(n)
+    g.n = n
+    cs = g.struct()
+    fl.check(fl.lib().fl_gen_cells(C.byref(cs), n, 42, 10, None))
+    tb = fl.lib().fl_regrid_temp_bytes(n)
+    t = fl.DeviceArray(tb, np.uint8)
+    fl.check(fl.lib().fl_gen_flags(C.byref(cs), n, 42, 0.05, 0.03, t.p, tb, None))
+    h = g.to_host()
+    ref = c3_cells(oracle, n, seed=42)
+    for nm in ("x", "y", "z", "level", "flag"):
+        assert np.array_equal(h[nm], getattr(ref, nm)), nm
+
+
+def test_amr_microbenchmark_full_size_properties(fl):
+    """64M cells: the regrid output is a permutation-free function of the
+    input counts (survivors + 8 x split + groups) and every balance pass
+    converged."""
+    from fluidloom_amd import amr_bench
+    r = amr_bench.run()
+    assert r["balance_converged"]
+    assert r["cells_out"] <= r["cells"] + 7 * r["split"]
+    assert r["merged_groups"] > 0 and r["split"] > 0

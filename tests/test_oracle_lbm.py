@@ -133,3 +133,36 @@ def test_sphere_case_runs_and_drags(oracle):
     rho, u = s.moments()
     assert np.isfinite(F).all() and F[0] > 0
     assert np.isfinite(u).all() and np.abs(rho - 1).max() < 0.1
+
+
+def test_cavity_regrid_refines_lid_corners_and_conserves(oracle):
+    from tests.cavity_util import OracleCavity
+    cav = OracleCavity(oracle, n0=16, max_level=1)
+    cav.run(121, 40)
+    s = cav.s
+    assert (s.level == 1).any() and (s.level == 0).any()
+    # refinement sits where the shear is: next to the moving lid
+    zc = (s.z + 0.5) / 2.0 ** s.level
+    assert zc[s.level == 1].mean() > zc[s.level == 0].mean()
+    rho, u = s.moments()
+    assert np.isfinite(u).all() and np.abs(rho - 1).max() < 0.05
+    # a regrid with unchanged leaves is the identity on the populations
+    f0 = s.get_f()
+    s.regrid(s.x, s.y, s.z, s.level, s.state)
+    assert np.array_equal(s.get_f(), f0)
+
+
+def test_regrid_split_and_merge_rules(oracle):
+    """Splitting a uniform-flow region and merging it back keeps the flow
+    uniform (equilibrium states carry no f_neq)."""
+    cs_u = (0.03, 0.0, 0.01)
+    c = uniform_cells(oracle, (8, 8, 8))
+    s = oracle.Lbm(8, 8, 8, 1, 0.7, [PER] * 6, ZERO, c.x, c.y, c.z, c.level, c.state, u_init=cs_u)
+    ref = refine_mesh(oracle, c, (8, 8, 8), [(1, box_region((2, 2, 2), (6, 6, 6)))], 1)
+    s.regrid(ref.x, ref.y, ref.z, ref.level, ref.state)
+    s.step(2)
+    rho, u = s.moments()
+    assert np.abs(u - np.array(cs_u)).max() < 1e-13
+    s.regrid(c.x, c.y, c.z, c.level, c.state)
+    rho, u = s.moments()
+    assert np.abs(u - np.array(cs_u)).max() < 1e-13 and np.abs(rho - 1).max() < 1e-13
