@@ -65,7 +65,8 @@ class LbmParams(C.Structure):
 class LbmStats(C.Structure):
     _fields_ = [("leaves", u64 * 24), ("bricks", u64 * 24), ("active_bricks", u64 * 24),
                 ("c2f", u64 * 24), ("f2c", u64 * 24), ("capture", u64 * 24), ("nlevels", i32),
-                ("pad", i32), ("updates_per_step", u64), ("device_bytes", u64)]
+                ("pad", i32), ("updates_per_step", u64), ("device_bytes", u64),
+                ("variant", i32 * 24), ("ninst", i32), ("graph", i32), ("graph_launches", u64)]
 
 
 _SIGS = {
@@ -139,6 +140,25 @@ _SIGS = {
     "fl_lbm_force": (C.c_int, [vp, vp]),
     "fl_lbm_stream": (C.c_int, [vp, C.POINTER(vp)]),
     "fl_lbm_time_collide": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+    "fl_lbm_create_batch": (C.c_int, [C.POINTER(LbmParams), C.c_int, vp, vp, vp, vp, vp, vp, vp, C.POINTER(vp)]),
+    "fl_lbm_plan_bytes": (C.c_int, [C.POINTER(LbmParams), C.c_int, vp, vp, vp, vp, vp, vp, C.POINTER(u64)]),
+    "fl_lbm_num_instances": (C.c_int, [vp, C.POINTER(i32)]),
+    "fl_lbm_inst_leaves": (C.c_int, [vp, C.c_int, vp, C.POINTER(u64)]),
+    "fl_lbm_get_f_inst": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, sz, vp]),
+    "fl_lbm_set_f_inst": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, sz, vp]),
+    "fl_lbm_moments_inst": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, sz, vp, vp]),
+    "fl_lbm_vorticity_inst": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, sz, vp]),
+    "fl_lbm_regrid_batch": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+    "fl_lbm_totals_batch": (C.c_int, [vp, vp, vp]),
+    "fl_lbm_force_batch": (C.c_int, [vp, vp]),
+    "fl_lbm_velocity_change": (C.c_int, [vp, vp]),
+    "fl_lbm_set_live": (C.c_int, [vp, C.c_int, C.c_int]),
+    "fl_lbm_set_graph": (C.c_int, [vp, C.c_int]),
+    "fl_lbm_num_variants": (C.c_int, []),
+    "fl_lbm_variant_name": (C.c_char_p, [C.c_int]),
+    "fl_lbm_set_variant": (C.c_int, [vp, C.c_int, C.c_int]),
+    "fl_lbm_autotune": (C.c_int, [vp, C.c_int, vp]),
+    "fl_lbm_time_collide_variant": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float)]),
 }
 
 _lib = None

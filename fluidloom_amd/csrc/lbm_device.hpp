@@ -102,29 +102,44 @@ __device__ inline void bgk(R* f, R inv_tau) {
 constexpr uint64_t kActiveBit = 1ull << 63;
 enum : unsigned { kLinkRegular = 0, kLinkSolid = 1, kLinkWall = 2, kLinkOutflow = 3 };
 
+// Per-instance constants (one record per instance of a batch).
+struct InstDev {
+    int32_t bc_type[6];   // faces -x,+x,-y,+y,-z,+z
+    float bc_u[6][3];
+    double bc_ud[6][3];
+    double rho0;          // density in the moving-wall term
+    double tau0;          // level-0 relaxation time
+    double rho_init;      // initial equilibrium
+    double u_init[3];
+    int32_t ext0[3];      // level-0 extents in cells
+    int32_t live;         // 0: the instance is not stepped any more
+};
+
 struct LevelDev {
     const int32_t* nbr;      // [brick][27], -1 = none
     const int32_t* bcoord;   // [brick][3]
     const uint32_t* bflags;  // bit 0: simple brick (all 64 active, all links regular)
     const uint64_t* cflags;  // [slot]
+    const uint16_t* binst;   // [brick] instance id
+    const float* itau;       // [instance] 1 / tau at this level
+    const double* itau_d;
+    const InstDev* inst;     // [instance]
     size_t S;                // slot stride = nbricks * 64
-    int32_t ext[3];          // domain extent in cells at this level
     int32_t nactive;         // active bricks (they come first)
+    int32_t level;
 };
 
-struct BCDev {
-    int32_t type[6];
-    float u[6][3];
-    double ud[6][3];
-    double rho0;
-};
+__host__ __device__ inline double tau_at(double tau0, int level) {
+    return (tau0 - 0.5) * (double)(1u << level) + 0.5;
+}
 
 // Wall face for an outside source (axes z, y, x in that order, WALL faces only).
-__device__ inline int wall_face(const BCDev& bc, const int32_t* ext, int sx, int sy, int sz) {
+__device__ inline int wall_face(const InstDev& I, int level, int sx, int sy, int sz) {
     int p[3] = {sx, sy, sz};
     for (int a = 2; a >= 0; --a) {
-        int f = p[a] < 0 ? 2 * a : (p[a] >= ext[a] ? 2 * a + 1 : -1);
-        if (f >= 0 && bc.type[f] == 0) return f;
+        const int e = I.ext0[a] << level;
+        int f = p[a] < 0 ? 2 * a : (p[a] >= e ? 2 * a + 1 : -1);
+        if (f >= 0 && I.bc_type[f] == 0) return f;
     }
     return -1;
 }
@@ -137,45 +152,51 @@ __device__ inline int64_t slot_of(const LevelDev& L, int b, int px, int py, int 
     return (int64_t)nb * 64 + (px & 3) + 4 * (py & 3) + 16 * (pz & 3);
 }
 
+// A non-regular link of direction i of `slot` (brick b, local offsets px..):
+// bounce-back from a solid, moving-wall bounce-back, or outflow copy.
+template <typename T, typename R>
+__device__ inline R special_link(const LevelDev& L, const T* __restrict__ A, int b, size_t slot, int i, unsigned code,
+                                 int px, int py, int pz) {
+    const size_t S = L.S;
+    if (code == kLinkSolid) return Store<T>::load(A, (size_t)opp(i) * S + slot, opp(i));
+    const InstDev& I = L.inst[L.binst[b]];
+    const int32_t* c3 = L.bcoord + 3 * b;
+    if (code == kLinkWall) {
+        const int face = wall_face(I, L.level, c3[0] * 4 + px, c3[1] * 4 + py, c3[2] * 4 + pz);
+        R uw = 0;
+        if (face >= 0) {
+            if (sizeof(R) == 8)
+                uw = (R)(cx(i) * I.bc_ud[face][0] + cy(i) * I.bc_ud[face][1] + cz(i) * I.bc_ud[face][2]);
+            else
+                uw = (R)((float)cx(i) * I.bc_u[face][0] + (float)cy(i) * I.bc_u[face][1] +
+                         (float)cz(i) * I.bc_u[face][2]);
+        }
+        return Store<T>::load(A, (size_t)opp(i) * S + slot, opp(i)) + (R)6 * (R)wq(i) * (R)I.rho0 * uw;
+    }
+    const int ex = I.ext0[0] << L.level, ey = I.ext0[1] << L.level, ez = I.ext0[2] << L.level;
+    const int gx = c3[0] * 4 + px, gy = c3[1] * 4 + py, gz = c3[2] * 4 + pz;
+    const int qx = gx < 0 ? 0 : (gx >= ex ? ex - 1 : gx);
+    const int qy = gy < 0 ? 0 : (gy >= ey ? ey - 1 : gy);
+    const int qz = gz < 0 ? 0 : (gz >= ez ? ez - 1 : gz);
+    const int64_t s2 = slot_of(L, b, qx - c3[0] * 4, qy - c3[1] * 4, qz - c3[2] * 4);
+    return Store<T>::load(A, (size_t)i * S + (size_t)s2, i);
+}
+
 // Generic pull of the 19 incoming populations of one slot (list kernels).
 template <typename T, typename R>
-__device__ inline void gather_slot(const LevelDev& L, const BCDev& bc, const T* __restrict__ A, int64_t slot,
-                                   R* f) {
+__device__ inline void gather_slot(const LevelDev& L, const T* __restrict__ A, int64_t slot, R* f) {
     const int b = (int)(slot >> 6);
     const int l = (int)(slot & 63);
     const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
     const uint64_t w = L.cflags[slot];
-    const size_t S = L.S;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         const unsigned code = (unsigned)(w >> (2 * i)) & 3u;
         const int px = lx - cx(i), py = ly - cy(i), pz = lz - cz(i);
-        if (code == kLinkRegular) {
-            f[i] = Store<T>::load(A, (size_t)i * S + (size_t)slot_of(L, b, px, py, pz), i);
-        } else if (code == kLinkSolid) {
-            f[i] = Store<T>::load(A, (size_t)opp(i) * S + (size_t)slot, opp(i));
-        } else if (code == kLinkWall) {
-            const int32_t* bc3 = L.bcoord + 3 * b;
-            int face = wall_face(bc, L.ext, bc3[0] * 4 + px, bc3[1] * 4 + py, bc3[2] * 4 + pz);
-            R uw = 0;
-            if (face >= 0) {
-                if (sizeof(R) == 8)
-                    uw = (R)(cx(i) * bc.ud[face][0] + cy(i) * bc.ud[face][1] + cz(i) * bc.ud[face][2]);
-                else
-                    uw = (R)((float)cx(i) * bc.u[face][0] + (float)cy(i) * bc.u[face][1] +
-                             (float)cz(i) * bc.u[face][2]);
-            }
-            f[i] = Store<T>::load(A, (size_t)opp(i) * S + (size_t)slot, opp(i)) +
-                   (R)6 * (R)wq(i) * (R)bc.rho0 * uw;
-        } else {
-            const int32_t* bc3 = L.bcoord + 3 * b;
-            int gx = bc3[0] * 4 + px, gy = bc3[1] * 4 + py, gz = bc3[2] * 4 + pz;
-            int qx = gx < 0 ? 0 : (gx >= L.ext[0] ? L.ext[0] - 1 : gx);
-            int qy = gy < 0 ? 0 : (gy >= L.ext[1] ? L.ext[1] - 1 : gy);
-            int qz = gz < 0 ? 0 : (gz >= L.ext[2] ? L.ext[2] - 1 : gz);
-            int64_t s2 = slot_of(L, b, qx - bc3[0] * 4, qy - bc3[1] * 4, qz - bc3[2] * 4);
-            f[i] = Store<T>::load(A, (size_t)i * S + (size_t)s2, i);
-        }
+        if (code == kLinkRegular)
+            f[i] = Store<T>::load(A, (size_t)i * L.S + (size_t)slot_of(L, b, px, py, pz), i);
+        else
+            f[i] = special_link<T, R>(L, A, b, (size_t)slot, i, code, px, py, pz);
     }
 }
 

@@ -1,4 +1,5 @@
-// a14/a15: D3Q19 lattice kernels for one refinement level.
+// a14/a15: D3Q19 lattice kernels for one refinement level of a batch of
+// instances (a single solve is a batch of one).
 //
 //   k_collide   fused pull-stream + BGK over the active bricks: one 64-lane
 //               wave per 4x4x4 brick, lane = cell.  The brick's 27-entry
@@ -8,10 +9,15 @@
 //               never read their link word.  Workgroups are remapped so that
 //               each XCD processes one contiguous Hilbert-ordered range of
 //               bricks, keeping cross-brick re-reads inside one L2.
+//               Variants: BPW = bricks (waves) per workgroup, with or
+//               without the XCD remap, or persistent (chosen per level by
+//               the solver's autotuner).
 //   k_capture   pre-collision state of coarse cells that feed c2f ghosts
 //   k_fill_c2f  coarse -> fine ghost populations (f_neq rescaled)
 //   k_fill_f2c  fine -> coarse ghost populations (8-child average)
-//   k_init, k_totals, k_force, k_extract: set-up, FP64 reductions, queries
+//   k_init, k_brick_sums, k_brick_du, k_inst_reduce, k_link_force,
+//   k_extract, k_inject, k_vorticity, k_remap_*, k_ghost_eq_*: set-up,
+//   deterministic FP64 reductions per instance, queries, regrid transfer.
 #include "fl_internal.hpp"
 #include "lbm_device.hpp"
 #include "lbm_kernels.hpp"
@@ -22,7 +28,6 @@ namespace lbm {
 namespace {
 
 constexpr int kWG = 256;
-constexpr int kBricksPerWG = kWG / 64;
 
 __device__ inline unsigned xcd_remap(unsigned bid, unsigned nblk) {
     const unsigned q = nblk / 8, r = nblk % 8;
@@ -30,13 +35,18 @@ __device__ inline unsigned xcd_remap(unsigned bid, unsigned nblk) {
     return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
+template <typename R>
+__device__ inline R inv_tau_of(const LevelDev& L, int inst) {
+    if (sizeof(R) == 8) return (R)L.itau_d[inst];
+    return (R)L.itau[inst];
+}
+
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_collide(LevelDev L, BCDev bc, const T* __restrict__ A, T* __restrict__ B,
-                                                 typename Store<T>::R inv_tau) {
+__device__ inline void collide_brick(const LevelDev& L, const T* __restrict__ A, T* __restrict__ B, int b) {
     using R = typename Store<T>::R;
-    const unsigned vb = xcd_remap(blockIdx.x, gridDim.x);
-    const int b = (int)(vb * kBricksPerWG + (threadIdx.x >> 6));
     if (b >= L.nactive) return;  // wave-uniform
+    const int inst = L.binst[b];
+    if (!L.inst[inst].live) return;
     const int lane = (int)(threadIdx.x & 63);
     const int nbrv = lane < 27 ? L.nbr[b * 27 + lane] : -1;
     const bool simple = (L.bflags[b] & 1u) != 0;
@@ -55,62 +65,72 @@ __global__ __launch_bounds__(kWG) void k_collide(LevelDev L, BCDev bc, const T* 
         f[i] = 0;
         if (active) {
             const unsigned code = (unsigned)(w >> (2 * i)) & 3u;
-            if (code == kLinkRegular) {
+            if (code == kLinkRegular)
                 f[i] = Store<T>::load(A, (size_t)i * S + src, i);
-            } else {
-                if (code == kLinkSolid) {
-                    f[i] = Store<T>::load(A, (size_t)opp(i) * S + slot, opp(i));
-                } else if (code == kLinkWall) {
-                    const int32_t* c3 = L.bcoord + 3 * b;
-                    int face = wall_face(bc, L.ext, c3[0] * 4 + px, c3[1] * 4 + py, c3[2] * 4 + pz);
-                    R uw = 0;
-                    if (face >= 0) {
-                        if (sizeof(R) == 8)
-                            uw = (R)(cx(i) * bc.ud[face][0] + cy(i) * bc.ud[face][1] + cz(i) * bc.ud[face][2]);
-                        else
-                            uw = (R)((float)cx(i) * bc.u[face][0] + (float)cy(i) * bc.u[face][1] +
-                                     (float)cz(i) * bc.u[face][2]);
-                    }
-                    f[i] = Store<T>::load(A, (size_t)opp(i) * S + slot, opp(i)) + (R)6 * (R)wq(i) * (R)bc.rho0 * uw;
-                } else {
-                    const int32_t* c3 = L.bcoord + 3 * b;
-                    int gx = c3[0] * 4 + px, gy = c3[1] * 4 + py, gz = c3[2] * 4 + pz;
-                    int qx = gx < 0 ? 0 : (gx >= L.ext[0] ? L.ext[0] - 1 : gx);
-                    int qy = gy < 0 ? 0 : (gy >= L.ext[1] ? L.ext[1] - 1 : gy);
-                    int qz = gz < 0 ? 0 : (gz >= L.ext[2] ? L.ext[2] - 1 : gz);
-                    int64_t s2 = slot_of(L, b, qx - c3[0] * 4, qy - c3[1] * 4, qz - c3[2] * 4);
-                    f[i] = Store<T>::load(A, (size_t)i * S + (size_t)s2, i);
-                }
-            }
+            else
+                f[i] = special_link<T, R>(L, A, b, slot, i, code, px, py, pz);
         }
     }
     if (!active) return;
-    bgk(f, inv_tau);
+    bgk(f, inv_tau_of<R>(L, inst));
 #pragma unroll
     for (int i = 0; i < Q; ++i) Store<T>::store(B, (size_t)i * S + slot, i, f[i]);
 }
 
+// MODE 0: workgroup g takes brick group g; 1: the same with the XCD-aware
+// remap; 2: persistent -- a grid sized to the resident capacity, each
+// workgroup walking a contiguous chunk of groups (chunks of one XCD adjacent).
+template <typename T, int BPW, int MODE>
+__global__ __launch_bounds__(64 * BPW) void k_collide(LevelDev L, const T* __restrict__ A, T* __restrict__ B,
+                                                      unsigned ngroups) {
+    const int wave = (int)(threadIdx.x >> 6);
+    if (MODE == 2) {
+        const unsigned wg = xcd_remap(blockIdx.x, gridDim.x);
+        const unsigned per = (ngroups + gridDim.x - 1) / gridDim.x;
+        const unsigned g0 = wg * per;
+        const unsigned g1 = g0 + per < ngroups ? g0 + per : ngroups;
+        for (unsigned g = g0; g < g1; ++g) collide_brick<T>(L, A, B, (int)(g * BPW) + wave);
+    } else {
+        const unsigned g = MODE == 1 ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+        collide_brick<T>(L, A, B, (int)(g * BPW) + wave);
+    }
+}
+
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_capture(LevelDev L, BCDev bc, const T* __restrict__ A,
+__global__ __launch_bounds__(kWG) void k_capture(LevelDev L, const T* __restrict__ A,
                                                  const uint32_t* __restrict__ cap_slots, uint32_t ncap,
                                                  typename Store<T>::R* __restrict__ cap) {
     using R = typename Store<T>::R;
     for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < ncap; k += gridDim.x * kWG) {
         R f[Q];
-        gather_slot<T, R>(L, bc, A, cap_slots[k], f);
+        gather_slot<T, R>(L, A, cap_slots[k], f);
 #pragma unroll
         for (int i = 0; i < Q; ++i) cap[(size_t)i * ncap + k] = f[i];
     }
 }
 
+// factor of the c2f fill for a ghost at level l of an instance with tau0
+__device__ inline double c2f_factor(double tau0, int l) {
+    const double tf = tau_at(tau0, l), tc = tau_at(tau0, l - 1);
+    return (1.0 - 1.0 / tf) * (tf / (2.0 * tc));
+}
+__device__ inline double f2c_factor(double tau0, int l) {
+    const double tc = tau_at(tau0, l), tf = tau_at(tau0, l + 1);
+    return (1.0 - 1.0 / tc) * (2.0 * tc / tf);
+}
+
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_fill_c2f(const uint32_t* __restrict__ ghost, const uint32_t* __restrict__ capidx,
-                                                  uint32_t n, const typename Store<T>::R* __restrict__ cap0,
+__global__ __launch_bounds__(kWG) void k_fill_c2f(LevelDev Lf, const uint32_t* __restrict__ ghost,
+                                                  const uint32_t* __restrict__ capidx, uint32_t n,
+                                                  const typename Store<T>::R* __restrict__ cap0,
                                                   const typename Store<T>::R* __restrict__ cap1, uint32_t ncap,
-                                                  typename Store<T>::R alpha, typename Store<T>::R fac,
-                                                  T* __restrict__ A, size_t S) {
+                                                  typename Store<T>::R alpha, T* __restrict__ A) {
     using R = typename Store<T>::R;
     for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < n; k += gridDim.x * kWG) {
+        const size_t g = ghost[k];
+        const InstDev& I = Lf.inst[Lf.binst[g >> 6]];
+        if (!I.live) continue;
+        const R fac = (R)c2f_factor(I.tau0, Lf.level);
         const uint32_t c = capidx[k];
         R f[Q];
 #pragma unroll
@@ -119,134 +139,206 @@ __global__ __launch_bounds__(kWG) void k_fill_c2f(const uint32_t* __restrict__ g
         R rho, ux, uy, uz;
         moments(f, rho, ux, uy, uz);
         const R usq = ux * ux + uy * uy + uz * uz;
-        const size_t g = ghost[k];
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const R e = feq(i, rho, ux, uy, uz, usq);
-            Store<T>::store(A, (size_t)i * S + g, i, e + fac * (f[i] - e));
+            Store<T>::store(A, (size_t)i * Lf.S + g, i, e + fac * (f[i] - e));
         }
     }
 }
 
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_fill_f2c(const uint32_t* __restrict__ ghost, const uint32_t* __restrict__ child8,
-                                                  uint32_t n, LevelDev Lf, BCDev bc, const T* __restrict__ Af,
-                                                  typename Store<T>::R fac, T* __restrict__ Ac, size_t Sc) {
+__device__ inline void child_average(const LevelDev& Lf, const T* __restrict__ Af, const uint32_t* c8,
+                                     typename Store<T>::R& r, typename Store<T>::R& ux, typename Store<T>::R& uy,
+                                     typename Store<T>::R& uz, typename Store<T>::R* aneq) {
+    using R = typename Store<T>::R;
+    R sr = 0, sjx = 0, sjy = 0, sjz = 0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) aneq[i] = 0;
+    for (int q = 0; q < 8; ++q) {
+        R f[Q];
+        gather_slot<T, R>(Lf, Af, c8[q], f);
+        R rho, vx, vy, vz;
+        moments(f, rho, vx, vy, vz);
+        const R usq = vx * vx + vy * vy + vz * vz;
+        sr += rho;
+        sjx += rho * vx;
+        sjy += rho * vy;
+        sjz += rho * vz;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) aneq[i] += f[i] - feq(i, rho, vx, vy, vz, usq);
+    }
+    r = sr * (R)0.125;
+    ux = (sjx * (R)0.125) / r;
+    uy = (sjy * (R)0.125) / r;
+    uz = (sjz * (R)0.125) / r;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) aneq[i] *= (R)0.125;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_fill_f2c(LevelDev Lc, const uint32_t* __restrict__ ghost,
+                                                  const uint32_t* __restrict__ child8, uint32_t n, LevelDev Lf,
+                                                  const T* __restrict__ Af, T* __restrict__ Ac) {
     using R = typename Store<T>::R;
     for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < n; k += gridDim.x * kWG) {
-        R sr = 0, sjx = 0, sjy = 0, sjz = 0, sneq[Q];
-#pragma unroll
-        for (int i = 0; i < Q; ++i) sneq[i] = 0;
-        for (int q = 0; q < 8; ++q) {
-            R f[Q];
-            gather_slot<T, R>(Lf, bc, Af, child8[(size_t)8 * k + q], f);
-            R rho, ux, uy, uz;
-            moments(f, rho, ux, uy, uz);
-            const R usq = ux * ux + uy * uy + uz * uz;
-            sr += rho;
-            sjx += rho * ux;
-            sjy += rho * uy;
-            sjz += rho * uz;
-#pragma unroll
-            for (int i = 0; i < Q; ++i) sneq[i] += f[i] - feq(i, rho, ux, uy, uz, usq);
-        }
-        const R r = sr * (R)0.125;
-        const R ux = (sjx * (R)0.125) / r, uy = (sjy * (R)0.125) / r, uz = (sjz * (R)0.125) / r;
-        const R usq = ux * ux + uy * uy + uz * uz;
         const size_t g = ghost[k];
+        const InstDev& I = Lc.inst[Lc.binst[g >> 6]];
+        if (!I.live) continue;
+        const R fac = (R)f2c_factor(I.tau0, Lc.level);
+        R r, ux, uy, uz, aneq[Q];
+        child_average<T>(Lf, Af, child8 + (size_t)8 * k, r, ux, uy, uz, aneq);
+        const R usq = ux * ux + uy * uy + uz * uz;
 #pragma unroll
         for (int i = 0; i < Q; ++i)
-            Store<T>::store(Ac, (size_t)i * Sc + g, i, feq(i, r, ux, uy, uz, usq) + fac * (sneq[i] * (R)0.125));
+            Store<T>::store(Ac, (size_t)i * Lc.S + g, i, feq(i, r, ux, uy, uz, usq) + fac * aneq[i]);
     }
 }
 
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_init(size_t S, double rho, double ux, double uy, double uz,
-                                              T* __restrict__ A, T* __restrict__ B) {
+__global__ __launch_bounds__(kWG) void k_init(LevelDev L, T* __restrict__ A, T* __restrict__ B) {
     using R = typename Store<T>::R;
-    const R usq = (R)(ux * ux + uy * uy + uz * uz);
-    for (size_t s = (size_t)blockIdx.x * kWG + threadIdx.x; s < S; s += (size_t)gridDim.x * kWG) {
+    for (size_t s = (size_t)blockIdx.x * kWG + threadIdx.x; s < L.S; s += (size_t)gridDim.x * kWG) {
+        const InstDev& I = L.inst[L.binst[s >> 6]];
+        const R ux = (R)I.u_init[0], uy = (R)I.u_init[1], uz = (R)I.u_init[2], rho = (R)I.rho_init;
+        const R usq = (R)(I.u_init[0] * I.u_init[0] + I.u_init[1] * I.u_init[1] + I.u_init[2] * I.u_init[2]);
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            R e = feq(i, (R)rho, (R)ux, (R)uy, (R)uz, usq);
-            Store<T>::store(A, (size_t)i * S + s, i, e);
-            Store<T>::store(B, (size_t)i * S + s, i, e);
+            R e = feq(i, rho, ux, uy, uz, usq);
+            Store<T>::store(A, (size_t)i * L.S + s, i, e);
+            Store<T>::store(B, (size_t)i * L.S + s, i, e);
         }
     }
 }
 
-// Per-block partial sums of (mass, jx, jy, jz) over active slots, in FP64.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// One wave per active brick: FP64 (mass, jx, jy, jz) * vol of its active
+// cells, summed in a fixed butterfly order.
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_totals(LevelDev L, const T* __restrict__ A, double vol,
-                                                double* __restrict__ partial) {
-    using R = typename Store<T>::R;
-    double acc[4] = {0, 0, 0, 0};
-    const size_t n = (size_t)L.nactive * 64;
-    for (size_t s = (size_t)blockIdx.x * kWG + threadIdx.x; s < n; s += (size_t)gridDim.x * kWG) {
-        const bool simple = (L.bflags[s >> 6] & 1u) != 0;
-        if (!simple && !(L.cflags[s] & kActiveBit)) continue;
-        double r = 0, jx = 0, jy = 0, jz = 0;
+__global__ __launch_bounds__(kWG) void k_brick_sums(LevelDev L, const T* __restrict__ A, double vol,
+                                                    double* __restrict__ out) {
+    const int b = (int)(blockIdx.x * (kWG / 64) + (threadIdx.x >> 6));
+    if (b >= L.nactive) return;
+    const size_t s = (size_t)b * 64 + (threadIdx.x & 63);
+    const bool act = (L.bflags[b] & 1u) != 0 || (L.cflags[s] & kActiveBit);
+    double r = 0, jx = 0, jy = 0, jz = 0;
+    if (act) {
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            double f = (double)Store<T>::load(A, (size_t)i * L.S + s, i);
+            const double f = (double)Store<T>::load(A, (size_t)i * L.S + s, i);
             r += f;
             jx += cx(i) * f;
             jy += cy(i) * f;
             jz += cz(i) * f;
         }
-        acc[0] += r * vol;
-        acc[1] += jx * vol;
-        acc[2] += jy * vol;
-        acc[3] += jz * vol;
     }
-    __shared__ double red[4][kWG];
-    for (int c = 0; c < 4; ++c) red[c][threadIdx.x] = acc[c];
-    __syncthreads();
-    for (int st = kWG / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st)
-            for (int c = 0; c < 4; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + st];
-        __syncthreads();
+    r = wave_sum(r * vol);
+    jx = wave_sum(jx * vol);
+    jy = wave_sum(jy * vol);
+    jz = wave_sum(jz * vol);
+    if ((threadIdx.x & 63) == 0) {
+        out[(size_t)b * 4 + 0] = r;
+        out[(size_t)b * 4 + 1] = jx;
+        out[(size_t)b * 4 + 2] = jy;
+        out[(size_t)b * 4 + 3] = jz;
     }
-    if (threadIdx.x == 0)
-        for (int c = 0; c < 4; ++c) partial[(size_t)blockIdx.x * 4 + c] = red[c][0];
 }
 
-// Momentum exchange over solid links (slot * 32 + q): F += -2 c_q f*_opp(q).
+// One wave per active brick: max over its active cells of |u - u_prev|
+// (infinity norm), u_prev replaced by u.
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_force(const uint32_t* __restrict__ links, uint32_t n, size_t S,
-                                               const T* __restrict__ A, double scale,
-                                               double* __restrict__ partial) {
-    double acc[3] = {0, 0, 0};
+__global__ __launch_bounds__(kWG) void k_brick_du(LevelDev L, const T* __restrict__ A, float* __restrict__ uprev,
+                                                  double* __restrict__ out) {
+    const int b = (int)(blockIdx.x * (kWG / 64) + (threadIdx.x >> 6));
+    if (b >= L.nactive) return;
+    const size_t s = (size_t)b * 64 + (threadIdx.x & 63);
+    const bool act = (L.bflags[b] & 1u) != 0 || (L.cflags[s] & kActiveBit);
+    double d = 0;
+    if (act) {
+        double r = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const double f = (double)Store<T>::load(A, (size_t)i * L.S + s, i);
+            r += f;
+            jx += cx(i) * f;
+            jy += cy(i) * f;
+            jz += cz(i) * f;
+        }
+        const float u[3] = {(float)(jx / r), (float)(jy / r), (float)(jz / r)};
+        for (int c = 0; c < 3; ++c) {
+            d = fmax(d, fabs((double)u[c] - (double)uprev[s * 3 + c]));
+            uprev[s * 3 + c] = u[c];
+        }
+    }
+    d = wave_max(d);
+    if ((threadIdx.x & 63) == 0) out[b] = d;
+}
+
+// One workgroup per instance: reduce the per-brick (or per-link) values of
+// the instance's range [beg[i], beg[i+1]) in a fixed order; op 0 = sum,
+// 1 = max.  first != 0 stores, else accumulates into acc[i * ncomp + c].
+__global__ __launch_bounds__(kWG) void k_inst_reduce(const double* __restrict__ vals, int ncomp,
+                                                     const uint32_t* __restrict__ beg, int op, int first,
+                                                     double* __restrict__ acc) {
+    const int inst = blockIdx.x;
+    const uint32_t b0 = beg[inst], b1 = beg[inst + 1];
+    __shared__ double red[kWG];
+    for (int c = 0; c < ncomp; ++c) {
+        double v = 0;
+        for (uint32_t k = b0 + threadIdx.x; k < b1; k += kWG) {
+            const double x = vals[(size_t)k * ncomp + c];
+            v = op ? fmax(v, x) : v + x;
+        }
+        red[threadIdx.x] = v;
+        __syncthreads();
+        for (int st = kWG / 2; st > 0; st >>= 1) {
+            if ((int)threadIdx.x < st)
+                red[threadIdx.x] = op ? fmax(red[threadIdx.x], red[threadIdx.x + st])
+                                      : red[threadIdx.x] + red[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            double& a = acc[(size_t)inst * ncomp + c];
+            a = first ? red[0] : (op ? fmax(a, red[0]) : a + red[0]);
+        }
+        __syncthreads();
+    }
+}
+
+// Momentum exchange per solid link (slot * 32 + q): -2 c_q f*_opp(q) * scale.
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_link_force(const uint32_t* __restrict__ links, uint32_t n, size_t S,
+                                                    const T* __restrict__ A, double scale, double* __restrict__ out) {
     for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < n; k += gridDim.x * kWG) {
         const uint32_t e = links[k];
         const size_t s = e >> 5;
         const int q = (int)(e & 31u);
         double fo = 0;
-        // q-dependent opposite load
-#pragma unroll
-        for (int i = 1; i < Q; ++i)
-            if (i == q) fo = (double)Store<T>::load(A, (size_t)opp(i) * S + s, opp(i));
         int c0 = 0, c1 = 0, c2 = 0;
 #pragma unroll
         for (int i = 1; i < Q; ++i)
-            if (i == q) { c0 = cx(i); c1 = cy(i); c2 = cz(i); }
-        acc[0] += -2.0 * c0 * fo * scale;
-        acc[1] += -2.0 * c1 * fo * scale;
-        acc[2] += -2.0 * c2 * fo * scale;
+            if (i == q) {
+                fo = (double)Store<T>::load(A, (size_t)opp(i) * S + s, opp(i));
+                c0 = cx(i);
+                c1 = cy(i);
+                c2 = cz(i);
+            }
+        out[(size_t)k * 3 + 0] = -2.0 * c0 * fo * scale;
+        out[(size_t)k * 3 + 1] = -2.0 * c1 * fo * scale;
+        out[(size_t)k * 3 + 2] = -2.0 * c2 * fo * scale;
     }
-    __shared__ double red[3][kWG];
-    for (int c = 0; c < 3; ++c) red[c][threadIdx.x] = acc[c];
-    __syncthreads();
-    for (int st = kWG / 2; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st)
-            for (int c = 0; c < 3; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + st];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        for (int c = 0; c < 3; ++c) partial[(size_t)blockIdx.x * 3 + c] = red[c][0];
 }
 
-// Populations of listed slots as FP64 [k][q].
 template <typename T>
 __global__ __launch_bounds__(kWG) void k_extract(const T* __restrict__ A, size_t S, const int64_t* __restrict__ slots,
                                                  size_t n, double* __restrict__ out) {
@@ -273,9 +365,7 @@ __global__ __launch_bounds__(kWG) void k_inject(T* __restrict__ A, T* __restrict
     }
 }
 
-// |vorticity| (level-0 units) at listed fluid-leaf slots: central
-// differences over same-level fluid-leaf neighbours, one-sided at the edge
-// of the level, zero when neither side is a fluid leaf.
+// |vorticity| (level-0 units) at listed fluid-leaf slots.
 template <typename T>
 __device__ inline bool leaf_velocity(const LevelDev& L, const T* __restrict__ A, int64_t s, double* u) {
     if (s < 0) return false;
@@ -338,14 +428,14 @@ __global__ __launch_bounds__(kWG) void k_remap_copy(const T* __restrict__ Ao, si
 }
 
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_remap_split(LevelDev Lc, BCDev bc, const T* __restrict__ Acprev,
+__global__ __launch_bounds__(kWG) void k_remap_split(LevelDev Ln, LevelDev Lc, const T* __restrict__ Acprev,
                                                      const uint32_t* __restrict__ dst, const uint32_t* __restrict__ src,
-                                                     uint32_t n, typename Store<T>::R fac, T* __restrict__ A,
-                                                     T* __restrict__ B, size_t S) {
+                                                     uint32_t n, T* __restrict__ A, T* __restrict__ B) {
     using R = typename Store<T>::R;
     for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < n; k += gridDim.x * kWG) {
+        const R fac = (R)c2f_factor(Ln.inst[Ln.binst[dst[k] >> 6]].tau0, Ln.level);
         R f[Q];
-        gather_slot<T, R>(Lc, bc, Acprev, src[k], f);
+        gather_slot<T, R>(Lc, Acprev, src[k], f);
         R rho, ux, uy, uz;
         moments(f, rho, ux, uy, uz);
         const R usq = ux * ux + uy * uy + uz * uz;
@@ -353,43 +443,27 @@ __global__ __launch_bounds__(kWG) void k_remap_split(LevelDev Lc, BCDev bc, cons
         for (int i = 0; i < Q; ++i) {
             const R e = feq(i, rho, ux, uy, uz, usq);
             const R v = e + fac * (f[i] - e);
-            Store<T>::store(A, (size_t)i * S + dst[k], i, v);
-            Store<T>::store(B, (size_t)i * S + dst[k], i, v);
+            Store<T>::store(A, (size_t)i * Ln.S + dst[k], i, v);
+            Store<T>::store(B, (size_t)i * Ln.S + dst[k], i, v);
         }
     }
 }
 
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_remap_merge(LevelDev Lf, BCDev bc, const T* __restrict__ Afprev,
+__global__ __launch_bounds__(kWG) void k_remap_merge(LevelDev Ln, LevelDev Lf, const T* __restrict__ Afprev,
                                                      const uint32_t* __restrict__ dst, const uint32_t* __restrict__ src8,
-                                                     uint32_t n, typename Store<T>::R fac, T* __restrict__ A,
-                                                     T* __restrict__ B, size_t S) {
+                                                     uint32_t n, T* __restrict__ A, T* __restrict__ B) {
     using R = typename Store<T>::R;
     for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < n; k += gridDim.x * kWG) {
-        R sr = 0, sjx = 0, sjy = 0, sjz = 0, sneq[Q];
-#pragma unroll
-        for (int i = 0; i < Q; ++i) sneq[i] = 0;
-        for (int q = 0; q < 8; ++q) {
-            R f[Q];
-            gather_slot<T, R>(Lf, bc, Afprev, src8[(size_t)8 * k + q], f);
-            R rho, ux, uy, uz;
-            moments(f, rho, ux, uy, uz);
-            const R usq = ux * ux + uy * uy + uz * uz;
-            sr += rho;
-            sjx += rho * ux;
-            sjy += rho * uy;
-            sjz += rho * uz;
-#pragma unroll
-            for (int i = 0; i < Q; ++i) sneq[i] += f[i] - feq(i, rho, ux, uy, uz, usq);
-        }
-        const R r = sr * (R)0.125;
-        const R ux = (sjx * (R)0.125) / r, uy = (sjy * (R)0.125) / r, uz = (sjz * (R)0.125) / r;
+        const R fac = (R)f2c_factor(Ln.inst[Ln.binst[dst[k] >> 6]].tau0, Ln.level);
+        R r, ux, uy, uz, aneq[Q];
+        child_average<T>(Lf, Afprev, src8 + (size_t)8 * k, r, ux, uy, uz, aneq);
         const R usq = ux * ux + uy * uy + uz * uz;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            const R v = feq(i, r, ux, uy, uz, usq) + fac * (sneq[i] * (R)0.125);
-            Store<T>::store(A, (size_t)i * S + dst[k], i, v);
-            Store<T>::store(B, (size_t)i * S + dst[k], i, v);
+            const R v = feq(i, r, ux, uy, uz, usq) + fac * aneq[i];
+            Store<T>::store(A, (size_t)i * Ln.S + dst[k], i, v);
+            Store<T>::store(B, (size_t)i * Ln.S + dst[k], i, v);
         }
     }
 }
@@ -470,66 +544,123 @@ unsigned lgrid(size_t n) {
     return (unsigned)(g ? g : 1);
 }
 
-}  // namespace
+int device_cus() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
 
-template <typename T>
-hipError_t launch_collide(const LevelDev& L, const BCDev& bc, const T* A, T* B, double inv_tau, hipStream_t s) {
-    if (L.nactive == 0) return hipSuccess;
-    unsigned g = (unsigned)((L.nactive + kBricksPerWG - 1) / kBricksPerWG);
-    hipLaunchKernelGGL(k_collide<T>, dim3(g), dim3(kWG), 0, s, L, bc, A, B, (typename Store<T>::R)inv_tau);
+template <typename T, int BPW, int MODE>
+hipError_t collide_variant(const LevelDev& L, const T* A, T* B, hipStream_t s) {
+    const unsigned ngroups = (unsigned)((L.nactive + BPW - 1) / BPW);
+    unsigned grid = ngroups;
+    if (MODE == 2) {
+        static int per_cu = 0;
+        if (per_cu == 0) {
+            int nb = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<T, BPW, MODE>, 64 * BPW, 0) !=
+                    hipSuccess ||
+                nb <= 0)
+                nb = 8 / BPW > 0 ? 8 / BPW : 1;
+            per_cu = nb;
+        }
+        const unsigned cap = (unsigned)(per_cu * device_cus());
+        if (grid > cap) grid = cap;
+    }
+    hipLaunchKernelGGL((k_collide<T, BPW, MODE>), dim3(grid), dim3(64 * BPW), 0, s, L, A, B, ngroups);
     return hipGetLastError();
 }
 
+}  // namespace
+
+int collide_variants() { return 9; }
+
+const char* collide_variant_name(int v) {
+    static const char* names[] = {"bpw1",     "bpw2",     "bpw4",         "bpw1-xcd",     "bpw2-xcd",
+                                  "bpw4-xcd", "bpw1-persist", "bpw2-persist", "bpw4-persist"};
+    return v >= 0 && v < 9 ? names[v] : "?";
+}
+
 template <typename T>
-hipError_t launch_capture(const LevelDev& L, const BCDev& bc, const T* A, const uint32_t* slots, uint32_t ncap,
-                          void* cap, hipStream_t s) {
+hipError_t launch_collide(const LevelDev& L, const T* A, T* B, int variant, hipStream_t s) {
+    if (L.nactive == 0) return hipSuccess;
+    switch (variant) {
+        case 0: return collide_variant<T, 1, 0>(L, A, B, s);
+        case 1: return collide_variant<T, 2, 0>(L, A, B, s);
+        case 2: return collide_variant<T, 4, 0>(L, A, B, s);
+        case 3: return collide_variant<T, 1, 1>(L, A, B, s);
+        case 4: return collide_variant<T, 2, 1>(L, A, B, s);
+        case 5: return collide_variant<T, 4, 1>(L, A, B, s);
+        case 6: return collide_variant<T, 1, 2>(L, A, B, s);
+        case 7: return collide_variant<T, 2, 2>(L, A, B, s);
+        default: return collide_variant<T, 4, 2>(L, A, B, s);
+    }
+}
+
+template <typename T>
+hipError_t launch_capture(const LevelDev& L, const T* A, const uint32_t* slots, uint32_t ncap, void* cap, hipStream_t s) {
     if (ncap == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_capture<T>, dim3(lgrid(ncap)), dim3(kWG), 0, s, L, bc, A, slots, ncap,
+    hipLaunchKernelGGL(k_capture<T>, dim3(lgrid(ncap)), dim3(kWG), 0, s, L, A, slots, ncap,
                        static_cast<typename Store<T>::R*>(cap));
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_fill_c2f(const uint32_t* ghost, const uint32_t* capidx, uint32_t n, const void* cap0,
-                           const void* cap1, uint32_t ncap, double alpha, double fac, T* A, size_t S,
-                           hipStream_t s) {
+hipError_t launch_fill_c2f(const LevelDev& Lf, const uint32_t* ghost, const uint32_t* capidx, uint32_t n,
+                           const void* cap0, const void* cap1, uint32_t ncap, double alpha, T* A, hipStream_t s) {
     if (n == 0) return hipSuccess;
     using R = typename Store<T>::R;
-    hipLaunchKernelGGL(k_fill_c2f<T>, dim3(lgrid(n)), dim3(kWG), 0, s, ghost, capidx, n,
-                       static_cast<const R*>(cap0), static_cast<const R*>(cap1), ncap, (R)alpha, (R)fac, A, S);
+    hipLaunchKernelGGL(k_fill_c2f<T>, dim3(lgrid(n)), dim3(kWG), 0, s, Lf, ghost, capidx, n,
+                       static_cast<const R*>(cap0), static_cast<const R*>(cap1), ncap, (R)alpha, A);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_fill_f2c(const uint32_t* ghost, const uint32_t* child8, uint32_t n, const LevelDev& Lf,
-                           const BCDev& bc, const T* Af, double fac, T* Ac, size_t Sc, hipStream_t s) {
+hipError_t launch_fill_f2c(const LevelDev& Lc, const uint32_t* ghost, const uint32_t* child8, uint32_t n,
+                           const LevelDev& Lf, const T* Af, T* Ac, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    using R = typename Store<T>::R;
-    hipLaunchKernelGGL(k_fill_f2c<T>, dim3(lgrid(n)), dim3(kWG), 0, s, ghost, child8, n, Lf, bc, Af, (R)fac, Ac, Sc);
+    hipLaunchKernelGGL(k_fill_f2c<T>, dim3(lgrid(n)), dim3(kWG), 0, s, Lc, ghost, child8, n, Lf, Af, Ac);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_init(size_t S, double rho, const double u[3], T* A, T* B, hipStream_t s) {
-    if (S == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_init<T>, dim3(lgrid(S)), dim3(kWG), 0, s, S, rho, u[0], u[1], u[2], A, B);
+hipError_t launch_init(const LevelDev& L, T* A, T* B, hipStream_t s) {
+    if (L.S == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_init<T>, dim3(lgrid(L.S)), dim3(kWG), 0, s, L, A, B);
     return hipGetLastError();
 }
 
-unsigned totals_blocks(const LevelDev& L) { return lgrid((size_t)L.nactive * 64) > 1024 ? 1024 : lgrid((size_t)L.nactive * 64); }
-
 template <typename T>
-hipError_t launch_totals(const LevelDev& L, const T* A, double vol, double* partial, unsigned nblk, hipStream_t s) {
-    hipLaunchKernelGGL(k_totals<T>, dim3(nblk), dim3(kWG), 0, s, L, A, vol, partial);
+hipError_t launch_brick_sums(const LevelDev& L, const T* A, double vol, double* out, hipStream_t s) {
+    if (L.nactive == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_brick_sums<T>, dim3((L.nactive + 3) / 4), dim3(kWG), 0, s, L, A, vol, out);
     return hipGetLastError();
 }
 
-unsigned force_blocks(uint32_t n) { return lgrid(n) > 1024 ? 1024 : lgrid(n); }
+template <typename T>
+hipError_t launch_brick_du(const LevelDev& L, const T* A, float* uprev, double* out, hipStream_t s) {
+    if (L.nactive == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_brick_du<T>, dim3((L.nactive + 3) / 4), dim3(kWG), 0, s, L, A, uprev, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_inst_reduce(const double* vals, int ncomp, const uint32_t* beg, int ninst, int op, int first,
+                              double* acc, hipStream_t s) {
+    if (ninst == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_inst_reduce, dim3(ninst), dim3(kWG), 0, s, vals, ncomp, beg, op, first, acc);
+    return hipGetLastError();
+}
 
 template <typename T>
-hipError_t launch_force(const uint32_t* links, uint32_t n, size_t S, const T* A, double scale, double* partial,
-                        unsigned nblk, hipStream_t s) {
-    hipLaunchKernelGGL(k_force<T>, dim3(nblk), dim3(kWG), 0, s, links, n, S, A, scale, partial);
+hipError_t launch_link_force(const uint32_t* links, uint32_t n, size_t S, const T* A, double scale, double* out,
+                             hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_link_force<T>, dim3(lgrid(n)), dim3(kWG), 0, s, links, n, S, A, scale, out);
     return hipGetLastError();
 }
 
@@ -559,15 +690,16 @@ template <typename T>
 hipError_t launch_remap(const RemapArgs<T>& r, hipStream_t s) {
     if (r.ncopy)
         hipLaunchKernelGGL(k_remap_copy<T>, dim3(lgrid(r.ncopy)), dim3(kWG), 0, s, r.copy_src_buf, r.So, r.copy_dst,
-                           r.copy_src, r.ncopy, r.A, r.B, r.S);
+                           r.copy_src, r.ncopy, r.A, r.B, r.Ln.S);
     if (r.nsplit)
-        hipLaunchKernelGGL(k_remap_split<T>, dim3(lgrid(r.nsplit)), dim3(kWG), 0, s, r.Lc, r.bc, r.split_prev,
-                           r.split_dst, r.split_src, r.nsplit, (typename Store<T>::R)r.split_fac, r.A, r.B, r.S);
+        hipLaunchKernelGGL(k_remap_split<T>, dim3(lgrid(r.nsplit)), dim3(kWG), 0, s, r.Ln, r.Lc, r.split_prev,
+                           r.split_dst, r.split_src, r.nsplit, r.A, r.B);
     if (r.nmerge)
-        hipLaunchKernelGGL(k_remap_merge<T>, dim3(lgrid(r.nmerge)), dim3(kWG), 0, s, r.Lf, r.bc, r.merge_prev,
-                           r.merge_dst, r.merge_src8, r.nmerge, (typename Store<T>::R)r.merge_fac, r.A, r.B, r.S);
+        hipLaunchKernelGGL(k_remap_merge<T>, dim3(lgrid(r.nmerge)), dim3(kWG), 0, s, r.Ln, r.Lf, r.merge_prev,
+                           r.merge_dst, r.merge_src8, r.nmerge, r.A, r.B);
     if (r.nrest)
-        hipLaunchKernelGGL(k_remap_rest<T>, dim3(lgrid(r.nrest)), dim3(kWG), 0, s, r.rest_dst, r.nrest, r.A, r.B, r.S);
+        hipLaunchKernelGGL(k_remap_rest<T>, dim3(lgrid(r.nrest)), dim3(kWG), 0, s, r.rest_dst, r.nrest, r.A, r.B,
+                           r.Ln.S);
     return hipGetLastError();
 }
 
@@ -584,23 +716,23 @@ hipError_t launch_ghost_eq(const uint32_t* c2f_ghost, const uint32_t* c2f_cap, c
     return hipGetLastError();
 }
 
-#define FL_INST(T)                                                                                           \
-    template hipError_t launch_collide<T>(const LevelDev&, const BCDev&, const T*, T*, double, hipStream_t); \
-    template hipError_t launch_capture<T>(const LevelDev&, const BCDev&, const T*, const uint32_t*, uint32_t, \
-                                          void*, hipStream_t);                                               \
-    template hipError_t launch_fill_c2f<T>(const uint32_t*, const uint32_t*, uint32_t, const void*, const void*, \
-                                           uint32_t, double, double, T*, size_t, hipStream_t);              \
-    template hipError_t launch_fill_f2c<T>(const uint32_t*, const uint32_t*, uint32_t, const LevelDev&,       \
-                                           const BCDev&, const T*, double, T*, size_t, hipStream_t);        \
-    template hipError_t launch_init<T>(size_t, double, const double*, T*, T*, hipStream_t);                  \
-    template hipError_t launch_totals<T>(const LevelDev&, const T*, double, double*, unsigned, hipStream_t); \
-    template hipError_t launch_force<T>(const uint32_t*, uint32_t, size_t, const T*, double, double*, unsigned, \
-                                        hipStream_t);                                                        \
-    template hipError_t launch_extract<T>(const T*, size_t, const int64_t*, size_t, double*, hipStream_t);   \
-    template hipError_t launch_inject<T>(T*, T*, size_t, const int64_t*, size_t, const double*, hipStream_t); \
-    template hipError_t launch_vorticity<T>(const LevelDev&, const T*, const int64_t*, size_t, double, double*,  \
-                                            hipStream_t);                                                    \
-    template hipError_t launch_remap<T>(const RemapArgs<T>&, hipStream_t);                                   \
+#define FL_INST(T)                                                                                            \
+    template hipError_t launch_collide<T>(const LevelDev&, const T*, T*, int, hipStream_t);                    \
+    template hipError_t launch_capture<T>(const LevelDev&, const T*, const uint32_t*, uint32_t, void*, hipStream_t); \
+    template hipError_t launch_fill_c2f<T>(const LevelDev&, const uint32_t*, const uint32_t*, uint32_t, const void*, \
+                                           const void*, uint32_t, double, T*, hipStream_t);                     \
+    template hipError_t launch_fill_f2c<T>(const LevelDev&, const uint32_t*, const uint32_t*, uint32_t,         \
+                                           const LevelDev&, const T*, T*, hipStream_t);                         \
+    template hipError_t launch_init<T>(const LevelDev&, T*, T*, hipStream_t);                                   \
+    template hipError_t launch_brick_sums<T>(const LevelDev&, const T*, double, double*, hipStream_t);          \
+    template hipError_t launch_brick_du<T>(const LevelDev&, const T*, float*, double*, hipStream_t);            \
+    template hipError_t launch_link_force<T>(const uint32_t*, uint32_t, size_t, const T*, double, double*,      \
+                                             hipStream_t);                                                      \
+    template hipError_t launch_extract<T>(const T*, size_t, const int64_t*, size_t, double*, hipStream_t);     \
+    template hipError_t launch_inject<T>(T*, T*, size_t, const int64_t*, size_t, const double*, hipStream_t);  \
+    template hipError_t launch_vorticity<T>(const LevelDev&, const T*, const int64_t*, size_t, double, double*, \
+                                            hipStream_t);                                                      \
+    template hipError_t launch_remap<T>(const RemapArgs<T>&, hipStream_t);                                     \
     template hipError_t launch_ghost_eq<T>(const uint32_t*, const uint32_t*, const uint32_t*, uint32_t, const T*, \
                                            size_t, const uint32_t*, const uint32_t*, uint32_t, const T*, size_t, \
                                            T*, T*, size_t, hipStream_t);

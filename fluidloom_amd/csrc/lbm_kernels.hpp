@@ -1,4 +1,5 @@
-// Host-side launchers of the lattice kernels (lbm_kernels.hip).
+// Host-side launchers of the lattice kernels (lbm_kernels.hip).  Every
+// launcher covers all instances of a batch at one level.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -9,27 +10,34 @@
 namespace fl {
 namespace lbm {
 
+// Collide-stream kernel variants (bricks per workgroup x XCD-aware remap).
+int collide_variants();
+const char* collide_variant_name(int v);
+
 template <typename T>
-hipError_t launch_collide(const LevelDev& L, const BCDev& bc, const T* A, T* B, double inv_tau, hipStream_t s);
+hipError_t launch_collide(const LevelDev& L, const T* A, T* B, int variant, hipStream_t s);
 template <typename T>
-hipError_t launch_capture(const LevelDev& L, const BCDev& bc, const T* A, const uint32_t* slots, uint32_t ncap,
-                          void* cap, hipStream_t s);
+hipError_t launch_capture(const LevelDev& L, const T* A, const uint32_t* slots, uint32_t ncap, void* cap, hipStream_t s);
 template <typename T>
-hipError_t launch_fill_c2f(const uint32_t* ghost, const uint32_t* capidx, uint32_t n, const void* cap0,
-                           const void* cap1, uint32_t ncap, double alpha, double fac, T* A, size_t S,
-                           hipStream_t s);
+hipError_t launch_fill_c2f(const LevelDev& Lf, const uint32_t* ghost, const uint32_t* capidx, uint32_t n,
+                           const void* cap0, const void* cap1, uint32_t ncap, double alpha, T* A, hipStream_t s);
 template <typename T>
-hipError_t launch_fill_f2c(const uint32_t* ghost, const uint32_t* child8, uint32_t n, const LevelDev& Lf,
-                           const BCDev& bc, const T* Af, double fac, T* Ac, size_t Sc, hipStream_t s);
+hipError_t launch_fill_f2c(const LevelDev& Lc, const uint32_t* ghost, const uint32_t* child8, uint32_t n,
+                           const LevelDev& Lf, const T* Af, T* Ac, hipStream_t s);
 template <typename T>
-hipError_t launch_init(size_t S, double rho, const double u[3], T* A, T* B, hipStream_t s);
-unsigned totals_blocks(const LevelDev& L);
+hipError_t launch_init(const LevelDev& L, T* A, T* B, hipStream_t s);
+// FP64 reductions: per-brick (or per-link) values, then one workgroup per
+// instance over its contiguous range (fixed order: results do not depend on
+// the other instances of the batch).
 template <typename T>
-hipError_t launch_totals(const LevelDev& L, const T* A, double vol, double* partial, unsigned nblk, hipStream_t s);
-unsigned force_blocks(uint32_t n);
+hipError_t launch_brick_sums(const LevelDev& L, const T* A, double vol, double* out, hipStream_t s);
 template <typename T>
-hipError_t launch_force(const uint32_t* links, uint32_t n, size_t S, const T* A, double scale, double* partial,
-                        unsigned nblk, hipStream_t s);
+hipError_t launch_brick_du(const LevelDev& L, const T* A, float* uprev, double* out, hipStream_t s);
+hipError_t launch_inst_reduce(const double* vals, int ncomp, const uint32_t* beg, int ninst, int op, int first,
+                              double* acc, hipStream_t s);
+template <typename T>
+hipError_t launch_link_force(const uint32_t* links, uint32_t n, size_t S, const T* A, double scale, double* out,
+                             hipStream_t s);
 template <typename T>
 hipError_t launch_extract(const T* A, size_t S, const int64_t* slots, size_t n, double* out, hipStream_t s);
 template <typename T>
@@ -41,9 +49,9 @@ hipError_t launch_vorticity(const LevelDev& L, const T* A, const int64_t* slots,
 // One level's population transfer after a regrid.
 template <typename T>
 struct RemapArgs {
+    LevelDev Ln{};                     // new level l
     T* A = nullptr;
     T* B = nullptr;
-    size_t S = 0;
     const T* copy_src_buf = nullptr;  // old level, current buffer
     size_t So = 0;
     const uint32_t* copy_dst = nullptr;
@@ -54,16 +62,13 @@ struct RemapArgs {
     const uint32_t* split_dst = nullptr;
     const uint32_t* split_src = nullptr;
     uint32_t nsplit = 0;
-    double split_fac = 0;
     LevelDev Lf{};                     // old level l+1
     const T* merge_prev = nullptr;
     const uint32_t* merge_dst = nullptr;
     const uint32_t* merge_src8 = nullptr;
     uint32_t nmerge = 0;
-    double merge_fac = 0;
     const uint32_t* rest_dst = nullptr;
     uint32_t nrest = 0;
-    BCDev bc{};
 };
 template <typename T>
 hipError_t launch_remap(const RemapArgs<T>& r, hipStream_t s);

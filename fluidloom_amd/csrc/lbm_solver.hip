@@ -21,6 +21,7 @@
 //    recursive sub-cycling; the coarse/fine exchange rescales f_neq as
 //    described in lbm_kernels.hip.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -42,10 +43,6 @@ namespace {
 enum : uint8_t { T_NONE = 0, T_FLUID = 1, T_SOLID = 2, T_C2F = 3, T_F2C = 4, T_WALL = 5 };
 
 inline bool regular(uint8_t t) { return t == T_FLUID || t == T_C2F || t == T_F2C; }
-
-inline uint64_t bkey(int32_t bx, int32_t by, int32_t bz) {
-    return ((uint64_t)(uint32_t)bx << 42) | ((uint64_t)(uint32_t)by << 21) | (uint64_t)(uint32_t)bz;
-}
 
 // Host copy of the 12-state table (src/core/hilbert/HilbertCodec.cpp:13-26),
 // used only to order bricks along the curve.
@@ -94,27 +91,45 @@ void parallel_for(size_t n, F&& f) {
     for (auto& t : th) t.join();
 }
 
+// Lattice parameters and leaf range of one instance of a batch.
+struct InstHost {
+    fl_lbm_params p{};
+    size_t first = 0, count = 0;
+};
+
+inline uint64_t bkey(uint32_t inst, int32_t bx, int32_t by, int32_t bz) {
+    return ((uint64_t)inst << 48) | ((uint64_t)(uint16_t)bx << 32) | ((uint64_t)(uint16_t)by << 16) |
+           (uint64_t)(uint16_t)bz;
+}
+inline uint32_t bkey_inst(uint64_t k) { return (uint32_t)(k >> 48); }
+inline int32_t bkey_c(uint64_t k, int a) { return (int32_t)((k >> (32 - 16 * a)) & 0xFFFF); }
+
 struct HostLevel {
-    int32_t ext[3] = {0, 0, 0};
-    int32_t nbx[3] = {0, 0, 0};
+    int level = 0;
+    std::vector<std::array<int32_t, 3>> iext;  // [instance] extents in cells at this level
     uint32_t nb = 0, nactive = 0;
     uint64_t nfluid = 0;
+    std::vector<uint64_t> nfluid_inst;
     std::unordered_map<uint64_t, int32_t> bmap;
     std::vector<int32_t> bcoord, nbr;
+    std::vector<uint16_t> binst;
     std::vector<uint32_t> bflags;
     std::vector<uint64_t> cflags;
     std::vector<uint8_t> stype, needed;
     std::vector<int64_t> c2fsrc;
     std::vector<uint32_t> c2f_ghost, c2f_cap, cap_slots, f2c_ghost, f2c_child, force_links;
+    std::vector<uint32_t> abeg;  // [ninst+1] active-brick range of each instance
+    std::vector<uint32_t> lbeg;  // [ninst+1] solid-link range of each instance
 
-    int64_t slot_at(int32_t x, int32_t y, int32_t z) const {
-        if (x < 0 || y < 0 || z < 0 || x >= ext[0] || y >= ext[1] || z >= ext[2]) return -1;
-        auto it = bmap.find(bkey(x >> 2, y >> 2, z >> 2));
+    int64_t slot_at(int inst, int32_t x, int32_t y, int32_t z) const {
+        const auto& e = iext[(size_t)inst];
+        if (x < 0 || y < 0 || z < 0 || x >= e[0] || y >= e[1] || z >= e[2]) return -1;
+        auto it = bmap.find(bkey((uint32_t)inst, x >> 2, y >> 2, z >> 2));
         if (it == bmap.end()) return -1;
         return (int64_t)it->second * 64 + (x & 3) + 4 * (y & 3) + 16 * (z & 3);
     }
-    uint8_t type_at(int32_t x, int32_t y, int32_t z) const {
-        int64_t s = slot_at(x, y, z);
+    uint8_t type_at(int inst, int32_t x, int32_t y, int32_t z) const {
+        int64_t s = slot_at(inst, x, y, z);
         return s < 0 ? T_NONE : stype[(size_t)s];
     }
 };
@@ -122,6 +137,7 @@ struct HostLevel {
 struct DevLevel {
     int32_t* nbr = nullptr;
     int32_t* bcoord = nullptr;
+    uint16_t* binst = nullptr;
     uint32_t* bflags = nullptr;
     uint64_t* cflags = nullptr;
     uint32_t* c2f_ghost = nullptr;
@@ -130,10 +146,15 @@ struct DevLevel {
     uint32_t* f2c_ghost = nullptr;
     uint32_t* f2c_child = nullptr;
     uint32_t* force_links = nullptr;
+    uint32_t* abeg = nullptr;
+    uint32_t* lbeg = nullptr;
+    float* itau = nullptr;
+    double* itau_d = nullptr;
+    double* bvals = nullptr;  // [max(nactive * 4, nlinks * 3)] reduction staging
+    float* uprev = nullptr;   // [nactive * 64 * 3] velocity snapshot (allocated on first use)
     void* buf[2] = {nullptr, nullptr};
     void* cap[2] = {nullptr, nullptr};
     int cur = 0;
-    double tau = 1.0;
     LevelDev ld{};
 };
 
@@ -145,22 +166,46 @@ size_t compute_bytes(int storage) { return storage == FL_STORE_F64 ? 8 : 4; }
 }  // namespace fl
 
 struct fl_lbm_solver {
-    fl_lbm_params p{};
+    std::vector<fl::lbm::InstHost> inst;
+    std::vector<fl::lbm::InstDev> hinst;
+    int storage = 0;
     hipStream_t stream = nullptr;
+    bool own_stream = false;
     int L = 0;
     fl::lbm::HostLevel h[24];
     fl::lbm::DevLevel d[24];
-    fl::lbm::BCDev bc{};
+    fl::lbm::InstDev* dinst = nullptr;
     std::vector<void*> allocs;
-    double* partial = nullptr;
+    double* acc = nullptr;  // [ninst * 4] per-instance reduction results
     int64_t* qslots = nullptr;
     double* qbuf = nullptr;
     size_t qcap = 0;
     uint64_t device_bytes = 0;
+    int variant[24];
+    bool use_graph = false;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    uint64_t graph_launches = 0;
 
-    ~fl_lbm_solver() {
-        for (void* a : allocs) (void)hipFree(a);
+    fl_lbm_solver() {
+        for (int l = 0; l < 24; ++l) variant[l] = 4;  // bpw2-xcd until tuned
     }
+    void drop_graphs() {
+        for (int k = 0; k < 2; ++k) {
+            if (gexec[k]) (void)hipGraphExecDestroy(gexec[k]);
+            if (graph[k]) (void)hipGraphDestroy(graph[k]);
+            gexec[k] = nullptr;
+            graph[k] = nullptr;
+        }
+    }
+    ~fl_lbm_solver() {
+        drop_graphs();
+        for (void* a : allocs) (void)hipFree(a);
+        if (qslots) (void)hipFree(qslots);
+        if (qbuf) (void)hipFree(qbuf);
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+    int ninst() const { return (int)inst.size(); }
 };
 
 namespace fl {
@@ -195,8 +240,8 @@ fl_status mesh_error(const std::string& m) {
 // Source of direction i from (x,y,z) after periodic wrap.  Returns 0 when
 // inside; else 1 with *face = wall face (axes z,y,x) or -1 for outflow, in
 // which case (sx,sy,sz) is clamped into the domain.
-int source_of(const fl_lbm_params& p, const int32_t* ext, int x, int y, int z, int i, int* sx, int* sy, int* sz,
-              int* face) {
+int source_of(const fl_lbm_params& p, const std::array<int32_t, 3>& ext, int x, int y, int z, int i, int* sx, int* sy,
+              int* sz, int* face) {
     int q[3] = {x - cx(i), y - cy(i), z - cz(i)};
     int out[3] = {0, 0, 0};
     bool any = false;
@@ -226,26 +271,46 @@ int source_of(const fl_lbm_params& p, const int32_t* ext, int x, int y, int z, i
     return 1;
 }
 
+// Brick coordinates of a neighbour after periodic wrap; false when outside.
+inline bool wrap_brick(const fl_lbm_params& p, const std::array<int32_t, 3>& nbx, int32_t* c) {
+    for (int a = 0; a < 3; ++a) {
+        if (c[a] >= 0 && c[a] < nbx[a]) continue;
+        int f = c[a] < 0 ? 2 * a : 2 * a + 1;
+        if (p.bc[f] != FL_BC_PERIODIC) return false;
+        c[a] = (c[a] + nbx[a]) % nbx[a];
+    }
+    return true;
+}
+
 fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, const int32_t* Z, const uint8_t* LV,
                        const uint8_t* ST, size_t n) {
-    const fl_lbm_params& p = S->p;
     const int NL = S->L;
-    // 1. leaf bricks per level
+    const int NI = S->ninst();
+    // leaf -> instance
+    std::vector<uint16_t> linst(n);
+    for (int k = 0; k < NI; ++k)
+        for (size_t j = S->inst[k].first; j < S->inst[k].first + S->inst[k].count; ++j) linst[j] = (uint16_t)k;
     for (int l = 0; l < NL; ++l) {
         HostLevel& H = S->h[l];
-        H.ext[0] = p.nx0 << l; H.ext[1] = p.ny0 << l; H.ext[2] = p.nz0 << l;
-        for (int a = 0; a < 3; ++a) H.nbx[a] = H.ext[a] / 4;
+        H.level = l;
+        H.iext.resize(NI);
+        for (int k = 0; k < NI; ++k) {
+            const fl_lbm_params& p = S->inst[k].p;
+            H.iext[k] = {p.nx0 << l, p.ny0 << l, p.nz0 << l};
+        }
     }
+    // 1. leaf bricks per level
     std::vector<std::vector<uint64_t>> active_keys(NL), leaf_keys(NL);
     {
         std::vector<uint64_t> last(NL, ~0ull), lastf(NL, ~0ull);
         for (size_t k = 0; k < n; ++k) {
-            int l = LV[k];
-            if (l >= NL) return mesh_error("leaf level above max_level");
-            const HostLevel& H = S->h[l];
-            if (X[k] < 0 || Y[k] < 0 || Z[k] < 0 || X[k] >= H.ext[0] || Y[k] >= H.ext[1] || Z[k] >= H.ext[2])
+            const int l = LV[k];
+            const int in = linst[k];
+            if (l > S->inst[in].p.max_level) return mesh_error("leaf level above the instance's max_level");
+            const auto& e = S->h[l].iext[in];
+            if (X[k] < 0 || Y[k] < 0 || Z[k] < 0 || X[k] >= e[0] || Y[k] >= e[1] || Z[k] >= e[2])
                 return mesh_error("leaf outside the domain");
-            uint64_t bk = bkey(X[k] >> 2, Y[k] >> 2, Z[k] >> 2);
+            uint64_t bk = bkey((uint32_t)in, X[k] >> 2, Y[k] >> 2, Z[k] >> 2);
             if (bk != last[l]) { leaf_keys[l].push_back(bk); last[l] = bk; }
             if (ST[k] == 0 && bk != lastf[l]) { active_keys[l].push_back(bk); lastf[l] = bk; }
         }
@@ -259,61 +324,56 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
         std::vector<uint64_t> all = leaf_keys[l];
         all.reserve(all.size() + ak.size() * 27);
         for (uint64_t k : ak) {
-            int32_t bx = (int32_t)(k >> 42), by = (int32_t)((k >> 21) & 0x1FFFFF), bz = (int32_t)(k & 0x1FFFFF);
+            const uint32_t in = bkey_inst(k);
+            const fl_lbm_params& p = S->inst[in].p;
+            const std::array<int32_t, 3> nbx = {H.iext[in][0] / 4, H.iext[in][1] / 4, H.iext[in][2] / 4};
             for (int dz = -1; dz <= 1; ++dz)
                 for (int dy = -1; dy <= 1; ++dy)
                     for (int dx = -1; dx <= 1; ++dx) {
-                        int32_t c[3] = {bx + dx, by + dy, bz + dz};
-                        bool ok = true;
-                        for (int a = 0; a < 3; ++a) {
-                            if (c[a] >= 0 && c[a] < H.nbx[a]) continue;
-                            int f = c[a] < 0 ? 2 * a : 2 * a + 1;
-                            if (p.bc[f] == FL_BC_PERIODIC) c[a] = (c[a] + H.nbx[a]) % H.nbx[a];
-                            else ok = false;
-                        }
-                        if (ok) all.push_back(bkey(c[0], c[1], c[2]));
+                        int32_t c[3] = {bkey_c(k, 0) + dx, bkey_c(k, 1) + dy, bkey_c(k, 2) + dz};
+                        if (wrap_brick(p, nbx, c)) all.push_back(bkey(in, c[0], c[1], c[2]));
                     }
         }
         std::sort(all.begin(), all.end());
         all.erase(std::unique(all.begin(), all.end()), all.end());
-        // order: active first, each group by Hilbert key of the brick coords
-        struct E { int act; uint64_t h; uint64_t k; };
+        // order: active first, then by instance, then along the Hilbert curve
+        struct E { int act; uint32_t inst; uint64_t h; uint64_t k; };
         std::vector<E> es(all.size());
         parallel_for(all.size(), [&](size_t i) {
             uint64_t k = all[i];
-            int32_t bx = (int32_t)(k >> 42), by = (int32_t)((k >> 21) & 0x1FFFFF), bz = (int32_t)(k & 0x1FFFFF);
-            es[i] = E{std::binary_search(ak.begin(), ak.end(), k) ? 0 : 1,
-                      host_encode((uint32_t)bx, (uint32_t)by, (uint32_t)bz, 21), k};
+            es[i] = E{std::binary_search(ak.begin(), ak.end(), k) ? 0 : 1, bkey_inst(k),
+                      host_encode((uint32_t)bkey_c(k, 0), (uint32_t)bkey_c(k, 1), (uint32_t)bkey_c(k, 2), 16), k};
         });
         std::sort(es.begin(), es.end(), [](const E& a, const E& b) {
-            return a.act != b.act ? a.act < b.act : a.h < b.h;
+            if (a.act != b.act) return a.act < b.act;
+            if (a.inst != b.inst) return a.inst < b.inst;
+            return a.h < b.h;
         });
         H.nb = (uint32_t)es.size();
         H.nactive = (uint32_t)ak.size();
         H.bcoord.resize((size_t)H.nb * 3);
+        H.binst.resize(H.nb);
         H.bmap.reserve(H.nb * 2);
+        H.abeg.assign(NI + 1, 0);
         for (uint32_t b = 0; b < H.nb; ++b) {
             uint64_t k = es[b].k;
-            H.bcoord[3 * b] = (int32_t)(k >> 42);
-            H.bcoord[3 * b + 1] = (int32_t)((k >> 21) & 0x1FFFFF);
-            H.bcoord[3 * b + 2] = (int32_t)(k & 0x1FFFFF);
+            for (int a = 0; a < 3; ++a) H.bcoord[3 * b + a] = bkey_c(k, a);
+            H.binst[b] = (uint16_t)bkey_inst(k);
             H.bmap.emplace(k, (int32_t)b);
+            if (b < H.nactive) H.abeg[bkey_inst(k) + 1]++;
         }
+        for (int k = 0; k < NI; ++k) H.abeg[k + 1] += H.abeg[k];
         H.nbr.assign((size_t)H.nb * 27, -1);
         parallel_for(H.nb, [&](size_t b) {
+            const uint32_t in = H.binst[b];
+            const fl_lbm_params& p = S->inst[in].p;
+            const std::array<int32_t, 3> nbx = {H.iext[in][0] / 4, H.iext[in][1] / 4, H.iext[in][2] / 4};
             for (int dz = -1; dz <= 1; ++dz)
                 for (int dy = -1; dy <= 1; ++dy)
                     for (int dx = -1; dx <= 1; ++dx) {
                         int32_t c[3] = {H.bcoord[3 * b] + dx, H.bcoord[3 * b + 1] + dy, H.bcoord[3 * b + 2] + dz};
-                        bool ok = true;
-                        for (int a = 0; a < 3; ++a) {
-                            if (c[a] >= 0 && c[a] < H.nbx[a]) continue;
-                            int f = c[a] < 0 ? 2 * a : 2 * a + 1;
-                            if (p.bc[f] == FL_BC_PERIODIC) c[a] = (c[a] + H.nbx[a]) % H.nbx[a];
-                            else ok = false;
-                        }
-                        if (!ok) continue;
-                        auto it = H.bmap.find(bkey(c[0], c[1], c[2]));
+                        if (!wrap_brick(p, nbx, c)) continue;
+                        auto it = H.bmap.find(bkey(in, c[0], c[1], c[2]));
                         if (it != H.bmap.end()) H.nbr[b * 27 + 13 + dx + 3 * dy + 9 * dz] = it->second;
                     }
         });
@@ -326,25 +386,35 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
     // 3. leaf slots
     {
         std::atomic<uint64_t> placed{0}, missing{0};
-        std::vector<std::atomic<uint64_t>> nfl(NL);
+        std::vector<std::atomic<uint64_t>> nfl((size_t)NL * NI);
         for (auto& a : nfl) a.store(0);
         parallel_for(n, [&](size_t k) {
             HostLevel& H = S->h[LV[k]];
-            int64_t s = H.slot_at(X[k], Y[k], Z[k]);
+            int64_t s = H.slot_at(linst[k], X[k], Y[k], Z[k]);
             if (s < 0) { missing.fetch_add(1); return; }
             uint8_t t = ST[k] == 0 ? T_FLUID : T_SOLID;
             uint8_t prev = __atomic_exchange_n(&H.stype[(size_t)s], t, __ATOMIC_RELAXED);
             if (prev == T_NONE) placed.fetch_add(1);
-            if (t == T_FLUID) nfl[LV[k]].fetch_add(1);
+            if (t == T_FLUID) nfl[(size_t)LV[k] * NI + linst[k]].fetch_add(1);
         });
         if (missing.load()) return mesh_error("internal: leaf brick missing");
         if (placed.load() != n) return mesh_error("duplicate leaf");
-        for (int l = 0; l < NL; ++l) S->h[l].nfluid = nfl[l].load();
+        for (int l = 0; l < NL; ++l) {
+            HostLevel& H = S->h[l];
+            H.nfluid_inst.assign(NI, 0);
+            H.nfluid = 0;
+            for (int k = 0; k < NI; ++k) {
+                H.nfluid_inst[k] = nfl[(size_t)l * NI + k].load();
+                H.nfluid += H.nfluid_inst[k];
+            }
+        }
     }
     // 4. classify the non-leaf slots from the leaves one level up / down
+    //    (reads only leaf kinds, which are final after step 3)
     for (int l = 0; l < NL; ++l) {
         HostLevel& H = S->h[l];
         parallel_for(H.nb, [&](size_t b) {
+            const int in = H.binst[b];
             for (int loc = 0; loc < 64; ++loc) {
                 size_t s = b * 64 + loc;
                 if (H.stype[s] != T_NONE) continue;
@@ -352,7 +422,7 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
                         z = H.bcoord[3 * b + 2] * 4 + (loc >> 4);
                 if (l > 0) {
                     const HostLevel& C = S->h[l - 1];
-                    int64_t cs = C.slot_at(x >> 1, y >> 1, z >> 1);
+                    int64_t cs = C.slot_at(in, x >> 1, y >> 1, z >> 1);
                     uint8_t ct = cs < 0 ? T_NONE : C.stype[(size_t)cs];
                     if (ct == T_FLUID) { H.stype[s] = T_C2F; H.c2fsrc[s] = cs; continue; }
                     if (ct == T_SOLID) { H.stype[s] = T_WALL; continue; }
@@ -361,7 +431,7 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
                     const HostLevel& F = S->h[l + 1];
                     int nf = 0, ns = 0;
                     for (int q = 0; q < 8; ++q) {
-                        uint8_t t = F.type_at(2 * x + (q & 1), 2 * y + ((q >> 1) & 1), 2 * z + ((q >> 2) & 1));
+                        uint8_t t = F.type_at(in, 2 * x + (q & 1), 2 * y + ((q >> 1) & 1), 2 * z + ((q >> 2) & 1));
                         nf += t == T_FLUID;
                         ns += t == T_SOLID;
                     }
@@ -371,16 +441,15 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
             }
         });
     }
-    // The f2c/c2f classification above read only leaf kinds of neighbouring
-    // levels (T_FLUID/T_SOLID are final before step 4 starts, and step 4
-    // only writes non-leaf kinds).
     // 5. link codes of every fluid leaf
     std::atomic<int> bad{0};
     for (int l = 0; l < NL; ++l) {
         HostLevel& H = S->h[l];
+        std::vector<std::vector<uint32_t>> blinks(H.nactive);
         parallel_for(H.nb, [&](size_t b) {
+            const int in = H.binst[b];
+            const fl_lbm_params& p = S->inst[in].p;
             bool simple = true;
-            std::vector<uint32_t> links;
             for (int loc = 0; loc < 64; ++loc) {
                 size_t s = b * 64 + loc;
                 if (H.stype[s] != T_FLUID) { simple = false; continue; }
@@ -390,11 +459,11 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
                 for (int i = 1; i < Q; ++i) {
                     int sx, sy, sz, face;
                     unsigned code;
-                    if (source_of(p, H.ext, x, y, z, i, &sx, &sy, &sz, &face)) {
+                    if (source_of(p, H.iext[in], x, y, z, i, &sx, &sy, &sz, &face)) {
                         if (face >= 0) {
                             code = kLinkWall;
                         } else {
-                            int64_t s2 = H.slot_at(sx, sy, sz);
+                            int64_t s2 = H.slot_at(in, sx, sy, sz);
                             uint8_t t2 = s2 < 0 ? T_NONE : H.stype[(size_t)s2];
                             if (regular(t2)) {
                                 code = kLinkOutflow;
@@ -416,7 +485,7 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
                             __atomic_store_n(&H.needed[(size_t)s2], (uint8_t)1, __ATOMIC_RELAXED);
                         } else if (t2 == T_SOLID || t2 == T_WALL) {
                             code = kLinkSolid;
-                            links.push_back((uint32_t)(s * 32 + i));
+                            blinks[b].push_back((uint32_t)(s * 32 + i));
                         } else {
                             bad.store(1);
                             code = kLinkSolid;
@@ -428,13 +497,15 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
                 H.cflags[s] = w;
             }
             H.bflags[b] = simple ? 1u : 0u;
-            if (!links.empty()) {
-                static std::mutex mu;
-                std::lock_guard<std::mutex> g(mu);
-                H.force_links.insert(H.force_links.end(), links.begin(), links.end());
-            }
         });
-        std::sort(H.force_links.begin(), H.force_links.end());
+        // solid links in slot order: contiguous per instance (active bricks
+        // are grouped by instance)
+        H.lbeg.assign(NI + 1, 0);
+        for (uint32_t b = 0; b < H.nactive; ++b) {
+            H.force_links.insert(H.force_links.end(), blinks[b].begin(), blinks[b].end());
+            H.lbeg[H.binst[b] + 1] += (uint32_t)blinks[b].size();
+        }
+        for (int k = 0; k < NI; ++k) H.lbeg[k + 1] += H.lbeg[k];
     }
     if (bad.load()) return mesh_error("a fluid leaf reads a position that is neither a leaf nor a ghost "
                                       "(mesh not 2:1 balanced over D3Q19 neighbour cubes)");
@@ -473,7 +544,7 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
                     z = H.bcoord[3 * b + 2] * 4 + (loc >> 4);
             for (int q = 0; q < 8; ++q)
                 H.f2c_child[8 * k + q] =
-                    (uint32_t)F.slot_at(2 * x + (q & 1), 2 * y + ((q >> 1) & 1), 2 * z + ((q >> 2) & 1));
+                    (uint32_t)F.slot_at(H.binst[b], 2 * x + (q & 1), 2 * y + ((q >> 1) & 1), 2 * z + ((q >> 2) & 1));
         }
     }
     for (int l = 0; l < NL; ++l)
@@ -482,17 +553,41 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
     return FL_OK;
 }
 
+// Device bytes the layout needs (buffers, tables, lists, staging).
+uint64_t layout_bytes(const fl_lbm_solver* S) {
+    const size_t sb = store_bytes(S->storage), cb = compute_bytes(S->storage);
+    const int NI = S->ninst();
+    uint64_t t = (uint64_t)NI * (sizeof(InstDev) + 4 * sizeof(double));
+    for (int l = 0; l < S->L; ++l) {
+        const HostLevel& H = S->h[l];
+        t += H.nbr.size() * 4 + H.bcoord.size() * 4 + H.binst.size() * 2 + H.bflags.size() * 4 + H.cflags.size() * 8;
+        t += (H.c2f_ghost.size() + H.c2f_cap.size() + H.cap_slots.size() + H.f2c_ghost.size() + H.f2c_child.size() +
+              H.force_links.size() + H.abeg.size() + H.lbeg.size()) * 4;
+        t += (uint64_t)NI * (sizeof(float) + sizeof(double));
+        t += std::max<uint64_t>((uint64_t)H.nactive * 4, (uint64_t)H.force_links.size() * 3) * 8;
+        t += 2ull * H.nb * 64 * Q * sb + 2ull * H.cap_slots.size() * Q * cb;
+    }
+    return t;
+}
+
 template <typename T>
 fl_status upload_and_init(fl_lbm_solver* S) {
     const size_t sb = sizeof(T);
-    const size_t cb = compute_bytes(S->p.storage);
-    size_t maxpart = 1024 * 4;
+    const size_t cb = compute_bytes(S->storage);
+    const int NI = S->ninst();
+    fl_status r;
+    if ((r = dev_upload(S, &S->dinst, S->hinst))) return r;
+    {
+        void* a;
+        if ((r = dev_alloc(S, &a, (size_t)NI * 4 * sizeof(double)))) return r;
+        S->acc = static_cast<double*>(a);
+    }
     for (int l = 0; l < S->L; ++l) {
         HostLevel& H = S->h[l];
         DevLevel& D = S->d[l];
-        fl_status r;
         if ((r = dev_upload(S, &D.nbr, H.nbr))) return r;
         if ((r = dev_upload(S, &D.bcoord, H.bcoord))) return r;
+        if ((r = dev_upload(S, &D.binst, H.binst))) return r;
         if ((r = dev_upload(S, &D.bflags, H.bflags))) return r;
         if ((r = dev_upload(S, &D.cflags, H.cflags))) return r;
         if ((r = dev_upload(S, &D.c2f_ghost, H.c2f_ghost))) return r;
@@ -501,28 +596,39 @@ fl_status upload_and_init(fl_lbm_solver* S) {
         if ((r = dev_upload(S, &D.f2c_ghost, H.f2c_ghost))) return r;
         if ((r = dev_upload(S, &D.f2c_child, H.f2c_child))) return r;
         if ((r = dev_upload(S, &D.force_links, H.force_links))) return r;
+        if ((r = dev_upload(S, &D.abeg, H.abeg))) return r;
+        if ((r = dev_upload(S, &D.lbeg, H.lbeg))) return r;
+        std::vector<float> itf(NI);
+        std::vector<double> itd(NI);
+        for (int k = 0; k < NI; ++k) {
+            itd[k] = 1.0 / tau_at(S->inst[k].p.tau0, l);
+            itf[k] = (float)itd[k];
+        }
+        if ((r = dev_upload(S, &D.itau, itf))) return r;
+        if ((r = dev_upload(S, &D.itau_d, itd))) return r;
+        void* bv;
+        const size_t nbv = std::max<size_t>((size_t)H.nactive * 4, H.force_links.size() * 3);
+        if ((r = dev_alloc(S, &bv, nbv * sizeof(double)))) return r;
+        D.bvals = static_cast<double*>(bv);
         const size_t slots = (size_t)H.nb * 64;
         for (int k = 0; k < 2; ++k) {
             if ((r = dev_alloc(S, &D.buf[k], slots * Q * sb))) return r;
             if ((r = dev_alloc(S, &D.cap[k], H.cap_slots.size() * Q * cb))) return r;
         }
-        D.tau = std::ldexp(S->p.tau0 - 0.5, l) + 0.5;
         D.cur = 0;
         D.ld.nbr = D.nbr;
         D.ld.bcoord = D.bcoord;
         D.ld.bflags = D.bflags;
         D.ld.cflags = D.cflags;
+        D.ld.binst = D.binst;
+        D.ld.itau = D.itau;
+        D.ld.itau_d = D.itau_d;
+        D.ld.inst = S->dinst;
         D.ld.S = slots;
-        for (int a = 0; a < 3; ++a) D.ld.ext[a] = H.ext[a];
         D.ld.nactive = (int32_t)H.nactive;
-        FL_HIP(launch_init<T>(slots, S->p.rho_init, S->p.u_init, static_cast<T*>(D.buf[0]),
-                              static_cast<T*>(D.buf[1]), S->stream));
-        maxpart = std::max(maxpart, (size_t)4 * 4096);
+        D.ld.level = l;
+        FL_HIP(launch_init<T>(D.ld, static_cast<T*>(D.buf[0]), static_cast<T*>(D.buf[1]), S->stream));
     }
-    void* part;
-    fl_status r = dev_alloc(S, &part, maxpart * sizeof(double));
-    if (r) return r;
-    S->partial = static_cast<double*>(part);
     FL_HIP(hipStreamSynchronize(S->stream));
     return FL_OK;
 }
@@ -536,27 +642,20 @@ fl_status advance(fl_lbm_solver* S, int l) {
     const uint32_t ncap = (uint32_t)H.cap_slots.size();
     if (fine) {
         DevLevel& F = S->d[l + 1];
-        const HostLevel& HF = S->h[l + 1];
-        double fac = (1.0 - 1.0 / D.tau) * (2.0 * D.tau / F.tau);
-        FL_HIP(launch_fill_f2c<T>(D.f2c_ghost, D.f2c_child, (uint32_t)H.f2c_ghost.size(), F.ld, S->bc,
-                                  static_cast<const T*>(F.buf[1 - F.cur]), fac, static_cast<T*>(D.buf[D.cur]),
-                                  D.ld.S, st));
-        (void)HF;
-        FL_HIP(launch_capture<T>(D.ld, S->bc, static_cast<const T*>(D.buf[1 - D.cur]), D.cap_slots, ncap, D.cap[0],
-                                 st));
+        FL_HIP(launch_fill_f2c<T>(D.ld, D.f2c_ghost, D.f2c_child, (uint32_t)H.f2c_ghost.size(), F.ld,
+                                  static_cast<const T*>(F.buf[1 - F.cur]), static_cast<T*>(D.buf[D.cur]), st));
+        FL_HIP(launch_capture<T>(D.ld, static_cast<const T*>(D.buf[1 - D.cur]), D.cap_slots, ncap, D.cap[0], st));
     }
-    FL_HIP(launch_collide<T>(D.ld, S->bc, static_cast<const T*>(D.buf[D.cur]), static_cast<T*>(D.buf[1 - D.cur]),
-                             1.0 / D.tau, st));
+    FL_HIP(launch_collide<T>(D.ld, static_cast<const T*>(D.buf[D.cur]), static_cast<T*>(D.buf[1 - D.cur]),
+                             S->variant[l], st));
     D.cur ^= 1;
     if (fine) {
-        FL_HIP(launch_capture<T>(D.ld, S->bc, static_cast<const T*>(D.buf[1 - D.cur]), D.cap_slots, ncap, D.cap[1],
-                                 st));
+        FL_HIP(launch_capture<T>(D.ld, static_cast<const T*>(D.buf[1 - D.cur]), D.cap_slots, ncap, D.cap[1], st));
         DevLevel& F = S->d[l + 1];
         const HostLevel& HF = S->h[l + 1];
-        double fac = (1.0 - 1.0 / F.tau) * (F.tau / (2.0 * D.tau));
         for (int sub = 0; sub < 2; ++sub) {
-            FL_HIP(launch_fill_c2f<T>(F.c2f_ghost, F.c2f_cap, (uint32_t)HF.c2f_ghost.size(), D.cap[0], D.cap[1], ncap,
-                                      0.5 * sub, fac, static_cast<T*>(F.buf[F.cur]), F.ld.S, st));
+            FL_HIP(launch_fill_c2f<T>(F.ld, F.c2f_ghost, F.c2f_cap, (uint32_t)HF.c2f_ghost.size(), D.cap[0], D.cap[1],
+                                      ncap, 0.5 * sub, static_cast<T*>(F.buf[F.cur]), st));
             fl_status r = advance<T>(S, l + 1);
             if (r) return r;
         }
@@ -564,29 +663,68 @@ fl_status advance(fl_lbm_solver* S, int l) {
     return FL_OK;
 }
 
+fl_status advance_any(fl_lbm_solver* S) {
+    switch (S->storage) {
+        case FL_STORE_F32: return advance<float>(S, 0);
+        case FL_STORE_F16: return advance<_Float16>(S, 0);
+        default: return advance<double>(S, 0);
+    }
+}
+
+// Two steps captured into a graph.  After two steps every level's buffer
+// parity is back where it started; levels >= 1 flip an even number of times
+// per step, so only level 0's parity selects the graph.
+fl_status capture_graph(fl_lbm_solver* S, int k) {
+    FL_HIP(hipStreamBeginCapture(S->stream, hipStreamCaptureModeRelaxed));
+    fl_status r = advance_any(S);
+    if (!r) r = advance_any(S);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(S->stream, &g);
+    if (r) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+    }
+    FL_HIP(e);
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        FL_HIP(e);
+    }
+    S->graph[k] = g;
+    S->gexec[k] = x;
+    return FL_OK;
+}
+
 fl_status dispatch_step(fl_lbm_solver* S, int nsteps) {
-    for (int t = 0; t < nsteps; ++t) {
-        fl_status r;
-        switch (S->p.storage) {
-            case FL_STORE_F32: r = advance<float>(S, 0); break;
-            case FL_STORE_F16: r = advance<_Float16>(S, 0); break;
-            default: r = advance<double>(S, 0); break;
+    int t = 0;
+    if (S->use_graph) {
+        for (; t + 2 <= nsteps; t += 2) {
+            const int k = S->d[0].cur;
+            if (!S->gexec[k]) {
+                fl_status r = capture_graph(S, k);
+                if (r) return r;
+            }
+            FL_HIP(hipGraphLaunch(S->gexec[k], S->stream));
+            ++S->graph_launches;
         }
+    }
+    for (; t < nsteps; ++t) {
+        fl_status r = advance_any(S);
         if (r) return r;
     }
     return FL_OK;
 }
 
-// host slots of the query leaves; -1 when not a fluid leaf of this mesh
-fl_status query_slots(fl_lbm_solver* S, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv,
-                      size_t n, std::vector<int64_t>& slots) {
+// host slots of the query leaves of one instance; -1 when not a fluid leaf
+void query_slots(fl_lbm_solver* S, int inst, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv,
+                 size_t n, std::vector<int64_t>& slots) {
     slots.resize(n);
     for (size_t k = 0; k < n; ++k) {
         int l = lv[k];
-        slots[k] = l < S->L ? S->h[l].slot_at(x[k], y[k], z[k]) : -1;
+        slots[k] = l < S->L ? S->h[l].slot_at(inst, x[k], y[k], z[k]) : -1;
         if (slots[k] >= 0 && S->h[l].stype[(size_t)slots[k]] != T_FLUID) slots[k] = -1;
     }
-    return FL_OK;
 }
 
 fl_status ensure_query(fl_lbm_solver* S, size_t n) {
@@ -595,6 +733,7 @@ fl_status ensure_query(fl_lbm_solver* S, size_t n) {
     if (S->qbuf) (void)hipFree(S->qbuf);
     S->qslots = nullptr;
     S->qbuf = nullptr;
+    S->qcap = 0;
     FL_HIP(hipMalloc(reinterpret_cast<void**>(&S->qslots), n * sizeof(int64_t)));
     FL_HIP(hipMalloc(reinterpret_cast<void**>(&S->qbuf), n * Q * sizeof(double)));
     S->qcap = n;
@@ -602,11 +741,14 @@ fl_status ensure_query(fl_lbm_solver* S, size_t n) {
 }
 
 template <typename T>
-fl_status get_f(fl_lbm_solver* S, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv, size_t n,
-                double* out) {
+fl_status get_f(fl_lbm_solver* S, int inst, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv,
+                size_t n, double* out) {
     std::vector<int64_t> slots;
-    query_slots(S, x, y, z, lv, n, slots);
+    query_slots(S, inst, x, y, z, lv, n, slots);
     std::vector<double> tmp;
+    for (size_t k = 0; k < n; ++k)
+        if (lv[k] >= S->L)
+            for (int i = 0; i < Q; ++i) out[k * Q + i] = std::nan("");
     for (int l = 0; l < S->L; ++l) {
         std::vector<int64_t> ls;
         std::vector<size_t> idx;
@@ -630,10 +772,10 @@ fl_status get_f(fl_lbm_solver* S, const int32_t* x, const int32_t* y, const int3
 }
 
 template <typename T>
-fl_status set_f(fl_lbm_solver* S, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv, size_t n,
-                const double* in) {
+fl_status set_f(fl_lbm_solver* S, int inst, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv,
+                size_t n, const double* in) {
     std::vector<int64_t> slots;
-    query_slots(S, x, y, z, lv, n, slots);
+    query_slots(S, inst, x, y, z, lv, n, slots);
     for (int l = 0; l < S->L; ++l) {
         std::vector<int64_t> ls;
         std::vector<double> vals;
@@ -655,78 +797,162 @@ fl_status set_f(fl_lbm_solver* S, const int32_t* x, const int32_t* y, const int3
     return FL_OK;
 }
 
+// Per-instance FP64 (mass, jx, jy, jz): per-brick wave sums, then one
+// workgroup per instance over its brick range, accumulated over levels.
 template <typename T>
-fl_status totals(fl_lbm_solver* S, double* mass, double* mom) {
-    double acc[4] = {0, 0, 0, 0};
-    std::vector<double> part;
+fl_status totals(fl_lbm_solver* S, double* out4) {
+    const int NI = S->ninst();
+    bool first = true;
     for (int l = 0; l < S->L; ++l) {
         DevLevel& D = S->d[l];
         if (D.ld.nactive == 0) continue;
-        unsigned nb = totals_blocks(D.ld);
-        FL_HIP(launch_totals<T>(D.ld, static_cast<const T*>(D.buf[D.cur]), std::ldexp(1.0, -3 * l), S->partial, nb,
-                                S->stream));
-        part.resize((size_t)nb * 4);
-        FL_HIP(hipMemcpyAsync(part.data(), S->partial, part.size() * sizeof(double), hipMemcpyDeviceToHost,
-                              S->stream));
-        FL_HIP(hipStreamSynchronize(S->stream));
-        for (unsigned b = 0; b < nb; ++b)
-            for (int c = 0; c < 4; ++c) acc[c] += part[(size_t)b * 4 + c];
+        FL_HIP(launch_brick_sums<T>(D.ld, static_cast<const T*>(D.buf[D.cur]), std::ldexp(1.0, -3 * l), D.bvals,
+                                    S->stream));
+        FL_HIP(launch_inst_reduce(D.bvals, 4, D.abeg, NI, 0, first ? 1 : 0, S->acc, S->stream));
+        first = false;
     }
-    *mass = acc[0];
-    mom[0] = acc[1]; mom[1] = acc[2]; mom[2] = acc[3];
+    if (first) {
+        for (int k = 0; k < 4 * NI; ++k) out4[k] = 0.0;
+        return FL_OK;
+    }
+    FL_HIP(hipMemcpyAsync(out4, S->acc, (size_t)NI * 4 * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+    FL_HIP(hipStreamSynchronize(S->stream));
     return FL_OK;
 }
 
 template <typename T>
-fl_status force(fl_lbm_solver* S, double* F) {
-    double acc[3] = {0, 0, 0};
-    std::vector<double> part;
+fl_status force(fl_lbm_solver* S, double* out3) {
+    const int NI = S->ninst();
+    bool first = true;
     for (int l = 0; l < S->L; ++l) {
         DevLevel& D = S->d[l];
-        uint32_t nl = (uint32_t)S->h[l].force_links.size();
+        const uint32_t nl = (uint32_t)S->h[l].force_links.size();
         if (nl == 0) continue;
-        unsigned nb = force_blocks(nl);
-        FL_HIP(launch_force<T>(D.force_links, nl, D.ld.S, static_cast<const T*>(D.buf[D.cur]),
-                               std::ldexp(1.0, -2 * l), S->partial, nb, S->stream));
-        part.resize((size_t)nb * 3);
-        FL_HIP(hipMemcpyAsync(part.data(), S->partial, part.size() * sizeof(double), hipMemcpyDeviceToHost,
-                              S->stream));
-        FL_HIP(hipStreamSynchronize(S->stream));
-        for (unsigned b = 0; b < nb; ++b)
-            for (int c = 0; c < 3; ++c) acc[c] += part[(size_t)b * 3 + c];
+        FL_HIP(launch_link_force<T>(D.force_links, nl, D.ld.S, static_cast<const T*>(D.buf[D.cur]),
+                                    std::ldexp(1.0, -2 * l), D.bvals, S->stream));
+        FL_HIP(launch_inst_reduce(D.bvals, 3, D.lbeg, NI, 0, first ? 1 : 0, S->acc, S->stream));
+        first = false;
     }
-    F[0] = acc[0]; F[1] = acc[1]; F[2] = acc[2];
+    if (first) {
+        for (int k = 0; k < 3 * NI; ++k) out3[k] = 0.0;
+        return FL_OK;
+    }
+    FL_HIP(hipMemcpyAsync(out3, S->acc, (size_t)NI * 3 * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+    FL_HIP(hipStreamSynchronize(S->stream));
     return FL_OK;
 }
 
-fl_status create_solver(const fl_lbm_params* p, const int32_t* h_x, const int32_t* h_y, const int32_t* h_z,
-                        const uint8_t* h_level, const uint8_t* h_state, size_t n, hipStream_t stream,
-                        std::unique_ptr<fl_lbm_solver>& out) {
-    FL_REQUIRE(n == 0 || (h_x && h_y && h_z && h_level && h_state), "null leaf arrays");
+template <typename T>
+fl_status velocity_change(fl_lbm_solver* S, double* du) {
+    const int NI = S->ninst();
+    bool fresh = false, first = true;
+    for (int l = 0; l < S->L; ++l) {
+        DevLevel& D = S->d[l];
+        if (D.ld.nactive == 0) continue;
+        if (!D.uprev) {
+            void* a;
+            fl_status r = dev_alloc(S, &a, (size_t)D.ld.nactive * 64 * 3 * sizeof(float));
+            if (r) return r;
+            D.uprev = static_cast<float*>(a);
+            fresh = true;
+        }
+        FL_HIP(launch_brick_du<T>(D.ld, static_cast<const T*>(D.buf[D.cur]), D.uprev, D.bvals, S->stream));
+        FL_HIP(launch_inst_reduce(D.bvals, 1, D.abeg, NI, 1, first ? 1 : 0, S->acc, S->stream));
+        first = false;
+    }
+    if (first) {
+        for (int k = 0; k < NI; ++k) du[k] = 0.0;
+        return FL_OK;
+    }
+    FL_HIP(hipMemcpyAsync(du, S->acc, (size_t)NI * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+    FL_HIP(hipStreamSynchronize(S->stream));
+    if (fresh)
+        for (int k = 0; k < NI; ++k) du[k] = HUGE_VAL;
+    return FL_OK;
+}
+
+fl_status check_params(const fl_lbm_params* p) {
     FL_REQUIRE(p->max_level >= 0 && p->max_level < 20, "max_level must be in 0..19");
     FL_REQUIRE(p->nx0 > 0 && p->ny0 > 0 && p->nz0 > 0 && p->nx0 % 4 == 0 && p->ny0 % 4 == 0 && p->nz0 % 4 == 0,
                "level-0 extents must be positive multiples of 4");
+    FL_REQUIRE(((int64_t)std::max(p->nx0, std::max(p->ny0, p->nz0)) << p->max_level) <= (int64_t)1 << 18,
+               "finest-level extent must not exceed 2^18 cells");
     FL_REQUIRE(p->tau0 > 0.5, "tau0 must exceed 1/2");
     FL_REQUIRE(p->storage >= 0 && p->storage <= 2, "unknown storage type");
     for (int f = 0; f < 6; ++f) FL_REQUIRE(p->bc[f] >= 0 && p->bc[f] <= 2, "unknown boundary type");
     for (int a = 0; a < 3; ++a)
         FL_REQUIRE((p->bc[2 * a] == FL_BC_PERIODIC) == (p->bc[2 * a + 1] == FL_BC_PERIODIC),
                    "periodic faces must come in pairs");
-    std::unique_ptr<fl_lbm_solver> S(new fl_lbm_solver());
-    S->p = *p;
-    S->L = p->max_level + 1;
-    S->stream = stream;
+    return FL_OK;
+}
+
+InstDev make_inst_dev(const fl_lbm_params& p) {
+    InstDev I{};
     for (int f = 0; f < 6; ++f) {
-        S->bc.type[f] = p->bc[f];
+        I.bc_type[f] = p.bc[f];
         for (int a = 0; a < 3; ++a) {
-            S->bc.u[f][a] = (float)p->bc_u[f][a];
-            S->bc.ud[f][a] = p->bc_u[f][a];
+            I.bc_u[f][a] = (float)p.bc_u[f][a];
+            I.bc_ud[f][a] = p.bc_u[f][a];
         }
     }
-    S->bc.rho0 = p->rho0;
+    I.rho0 = p.rho0;
+    I.tau0 = p.tau0;
+    I.rho_init = p.rho_init;
+    for (int a = 0; a < 3; ++a) I.u_init[a] = p.u_init[a];
+    I.ext0[0] = p.nx0;
+    I.ext0[1] = p.ny0;
+    I.ext0[2] = p.nz0;
+    I.live = 1;
+    return I;
+}
+
+// Host half of creation: validation and the layout (no device work).
+fl_status plan_solver(const fl_lbm_params* params, int ninst, const size_t* offsets, const int32_t* h_x,
+                      const int32_t* h_y, const int32_t* h_z, const uint8_t* h_level, const uint8_t* h_state,
+                      std::unique_ptr<fl_lbm_solver>& out) {
+    FL_REQUIRE(params && offsets, "null pointer");
+    FL_REQUIRE(ninst >= 1 && ninst <= 65535, "instance count must be in 1..65535");
+    const size_t n = offsets[ninst];
+    FL_REQUIRE(offsets[0] == 0, "offsets[0] must be 0");
+    FL_REQUIRE(n == 0 || (h_x && h_y && h_z && h_level && h_state), "null leaf arrays");
+    std::unique_ptr<fl_lbm_solver> S(new fl_lbm_solver());
+    S->storage = params[0].storage;
+    int maxl = 0;
+    for (int k = 0; k < ninst; ++k) {
+        fl_status r = check_params(&params[k]);
+        if (r) return r;
+        FL_REQUIRE(params[k].storage == S->storage, "every instance of a batch must use the same storage");
+        FL_REQUIRE(offsets[k + 1] >= offsets[k], "offsets must be non-decreasing");
+        InstHost ih;
+        ih.p = params[k];
+        ih.first = offsets[k];
+        ih.count = offsets[k + 1] - offsets[k];
+        S->inst.push_back(ih);
+        S->hinst.push_back(make_inst_dev(params[k]));
+        maxl = std::max(maxl, params[k].max_level);
+    }
+    S->L = maxl + 1;
     fl_status r = build_layout(S.get(), h_x, h_y, h_z, h_level, h_state, n);
     if (r) return r;
-    switch (p->storage) {
+    out = std::move(S);
+    return FL_OK;
+}
+
+fl_status create_solver(const fl_lbm_params* params, int ninst, const size_t* offsets, const int32_t* h_x,
+                        const int32_t* h_y, const int32_t* h_z, const uint8_t* h_level, const uint8_t* h_state,
+                        hipStream_t stream, std::unique_ptr<fl_lbm_solver>& out) {
+    std::unique_ptr<fl_lbm_solver> S;
+    fl_status r = plan_solver(params, ninst, offsets, h_x, h_y, h_z, h_level, h_state, S);
+    if (r) return r;
+    if (stream) {
+        S->stream = stream;
+    } else {
+        // an own stream so that the step can be graph-captured (the null
+        // stream cannot be); it synchronises with the null stream
+        FL_HIP(hipStreamCreate(&S->stream));
+        S->own_stream = true;
+    }
+    switch (S->storage) {
         case FL_STORE_F32: r = upload_and_init<float>(S.get()); break;
         case FL_STORE_F16: r = upload_and_init<_Float16>(S.get()); break;
         default: r = upload_and_init<double>(S.get()); break;
@@ -737,10 +963,10 @@ fl_status create_solver(const fl_lbm_params* p, const int32_t* h_x, const int32_
 }
 
 template <typename T>
-fl_status vorticity(fl_lbm_solver* S, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv,
-                    size_t n, double* out) {
+fl_status vorticity(fl_lbm_solver* S, int inst, const int32_t* x, const int32_t* y, const int32_t* z,
+                    const uint8_t* lv, size_t n, double* out) {
     std::vector<int64_t> slots;
-    query_slots(S, x, y, z, lv, n, slots);
+    query_slots(S, inst, x, y, z, lv, n, slots);
     for (size_t k = 0; k < n; ++k) out[k] = 0.0;
     std::vector<double> tmp;
     for (int l = 0; l < S->L; ++l) {
@@ -763,38 +989,49 @@ fl_status vorticity(fl_lbm_solver* S, const int32_t* x, const int32_t* y, const 
     return FL_OK;
 }
 
-// Population transfer from the old solver O into the freshly built N.
+// Population transfer from the old solver O into the freshly built N (same
+// instances, new leaves).
 template <typename T>
 fl_status remap(fl_lbm_solver* O, fl_lbm_solver* N) {
     std::vector<void*> tmp_allocs;
+    fl_status st = FL_OK;
     auto up = [&](const std::vector<uint32_t>& v) -> const uint32_t* {
-        if (v.empty()) return nullptr;
+        if (v.empty() || st) return nullptr;
         void* d = nullptr;
-        if (hipMalloc(&d, v.size() * 4) != hipSuccess) return nullptr;
+        hipError_t e = hipMalloc(&d, v.size() * 4);
+        if (e != hipSuccess) {
+            st = fl::hip_fail(e, "remap lists", __FILE__, __LINE__);
+            return nullptr;
+        }
         tmp_allocs.push_back(d);
-        (void)hipMemcpyAsync(d, v.data(), v.size() * 4, hipMemcpyHostToDevice, N->stream);
+        e = hipMemcpyAsync(d, v.data(), v.size() * 4, hipMemcpyHostToDevice, N->stream);
+        if (e != hipSuccess) st = fl::hip_fail(e, "remap lists", __FILE__, __LINE__);
         return static_cast<const uint32_t*>(d);
     };
-    fl_status st = FL_OK;
+    // host lists must outlive the asynchronous copies
+    std::vector<std::vector<uint32_t>> keep;
     for (int l = 0; l < N->L && st == FL_OK; ++l) {
         const HostLevel& H = N->h[l];
         std::vector<uint32_t> cdst, csrc, sdst, ssrc, mdst, msrc, rdst;
-        for (uint32_t b = 0; b < H.nb; ++b)
+        for (uint32_t b = 0; b < H.nactive; ++b) {
+            const int in = H.binst[b];
             for (int loc = 0; loc < 64; ++loc) {
                 const size_t s = (size_t)b * 64 + loc;
                 if (H.stype[s] != T_FLUID) continue;
                 const int32_t x = H.bcoord[3 * b] * 4 + (loc & 3), y = H.bcoord[3 * b + 1] * 4 + ((loc >> 2) & 3),
                               z = H.bcoord[3 * b + 2] * 4 + (loc >> 4);
-                const HostLevel& OH = O->h[l];
-                int64_t os = OH.slot_at(x, y, z);
-                if (os >= 0 && OH.stype[(size_t)os] == T_FLUID) {
-                    cdst.push_back((uint32_t)s);
-                    csrc.push_back((uint32_t)os);
-                    continue;
+                if (l < O->L) {
+                    const HostLevel& OH = O->h[l];
+                    int64_t os = OH.slot_at(in, x, y, z);
+                    if (os >= 0 && OH.stype[(size_t)os] == T_FLUID) {
+                        cdst.push_back((uint32_t)s);
+                        csrc.push_back((uint32_t)os);
+                        continue;
+                    }
                 }
-                if (l > 0) {
+                if (l > 0 && l - 1 < O->L) {
                     const HostLevel& OC = O->h[l - 1];
-                    int64_t ps = OC.slot_at(x >> 1, y >> 1, z >> 1);
+                    int64_t ps = OC.slot_at(in, x >> 1, y >> 1, z >> 1);
                     if (ps >= 0 && OC.stype[(size_t)ps] == T_FLUID) {
                         sdst.push_back((uint32_t)s);
                         ssrc.push_back((uint32_t)ps);
@@ -806,7 +1043,7 @@ fl_status remap(fl_lbm_solver* O, fl_lbm_solver* N) {
                     uint32_t ch[8];
                     bool all = true;
                     for (int q = 0; q < 8 && all; ++q) {
-                        int64_t c = OF.slot_at(2 * x + (q & 1), 2 * y + ((q >> 1) & 1), 2 * z + ((q >> 2) & 1));
+                        int64_t c = OF.slot_at(in, 2 * x + (q & 1), 2 * y + ((q >> 1) & 1), 2 * z + ((q >> 2) & 1));
                         all = c >= 0 && OF.stype[(size_t)c] == T_FLUID;
                         if (all) ch[q] = (uint32_t)c;
                     }
@@ -818,23 +1055,24 @@ fl_status remap(fl_lbm_solver* O, fl_lbm_solver* N) {
                 }
                 rdst.push_back((uint32_t)s);
             }
+        }
         RemapArgs<T> a;
         DevLevel& D = N->d[l];
+        a.Ln = D.ld;
         a.A = static_cast<T*>(D.buf[0]);
         a.B = static_cast<T*>(D.buf[1]);
-        a.S = D.ld.S;
-        a.bc = N->bc;
-        const DevLevel& OD = O->d[l];
-        a.copy_src_buf = static_cast<const T*>(OD.buf[OD.cur]);
-        a.So = OD.ld.S;
+        if (l < O->L) {
+            const DevLevel& OD = O->d[l];
+            a.copy_src_buf = static_cast<const T*>(OD.buf[OD.cur]);
+            a.So = OD.ld.S;
+        }
         a.copy_dst = up(cdst);
         a.copy_src = up(csrc);
         a.ncopy = (uint32_t)cdst.size();
-        if (l > 0) {
+        if (l > 0 && l - 1 < O->L) {
             const DevLevel& OC = O->d[l - 1];
             a.Lc = OC.ld;
             a.split_prev = static_cast<const T*>(OC.buf[1 - OC.cur]);
-            a.split_fac = (1.0 - 1.0 / D.tau) * (D.tau / (2.0 * OC.tau));
         }
         a.split_dst = up(sdst);
         a.split_src = up(ssrc);
@@ -843,16 +1081,23 @@ fl_status remap(fl_lbm_solver* O, fl_lbm_solver* N) {
             const DevLevel& OF = O->d[l + 1];
             a.Lf = OF.ld;
             a.merge_prev = static_cast<const T*>(OF.buf[1 - OF.cur]);
-            a.merge_fac = (1.0 - 1.0 / D.tau) * (2.0 * D.tau / OF.tau);
         }
         a.merge_dst = up(mdst);
         a.merge_src8 = up(msrc);
         a.nmerge = (uint32_t)mdst.size();
         a.rest_dst = up(rdst);
         a.nrest = (uint32_t)rdst.size();
+        if (st) break;
         hipError_t e = launch_remap<T>(a, N->stream);
         if (e != hipSuccess) st = fl::hip_fail(e, "launch_remap", __FILE__, __LINE__);
         D.cur = 0;
+        keep.push_back(std::move(cdst));
+        keep.push_back(std::move(csrc));
+        keep.push_back(std::move(sdst));
+        keep.push_back(std::move(ssrc));
+        keep.push_back(std::move(mdst));
+        keep.push_back(std::move(msrc));
+        keep.push_back(std::move(rdst));
     }
     for (int l = 0; l < N->L && st == FL_OK; ++l) {
         const HostLevel& H = N->h[l];
@@ -874,6 +1119,37 @@ fl_status remap(fl_lbm_solver* O, fl_lbm_solver* N) {
     return FL_OK;
 }
 
+template <typename T>
+fl_status time_collide(fl_lbm_solver* S, int level, int variant, int iters, float* ms) {
+    DevLevel& D = S->d[level];
+    hipEvent_t a, b;
+    FL_HIP(hipEventCreate(&a));
+    FL_HIP(hipEventCreate(&b));
+    hipError_t e = hipEventRecord(a, S->stream);
+    for (int it = 0; it < iters && e == hipSuccess; ++it) {
+        e = launch_collide<T>(D.ld, static_cast<const T*>(D.buf[D.cur]), static_cast<T*>(D.buf[1 - D.cur]), variant,
+                              S->stream);
+        D.cur ^= 1;
+    }
+    if (e == hipSuccess) e = hipEventRecord(b, S->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float t = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    FL_HIP(e);
+    *ms = t / iters;
+    return FL_OK;
+}
+
+fl_status time_any(fl_lbm_solver* S, int level, int variant, int iters, float* ms) {
+    switch (S->storage) {
+        case FL_STORE_F32: return time_collide<float>(S, level, variant, iters, ms);
+        case FL_STORE_F16: return time_collide<_Float16>(S, level, variant, iters, ms);
+        default: return time_collide<double>(S, level, variant, iters, ms);
+    }
+}
+
 }  // namespace
 }  // namespace lbm
 }  // namespace fl
@@ -881,25 +1157,46 @@ fl_status remap(fl_lbm_solver* O, fl_lbm_solver* N) {
 using namespace fl;
 using namespace fl::lbm;
 
+#define FL_SOLVER_INST(S, inst) \
+    FL_REQUIRE(S, "null solver"); \
+    FL_REQUIRE((inst) >= 0 && (inst) < (S)->ninst(), "instance index out of range")
+
 extern "C" {
 
 fl_status fl_lbm_create(const fl_lbm_params* p, const int32_t* h_x, const int32_t* h_y, const int32_t* h_z,
                         const uint8_t* h_level, const uint8_t* h_state, size_t n, fl_stream s,
                         fl_lbm_solver** out) {
     FL_REQUIRE(p && out, "null pointer");
+    const size_t off[2] = {0, n};
+    return fl_lbm_create_batch(p, 1, off, h_x, h_y, h_z, h_level, h_state, s, out);
+}
+
+fl_status fl_lbm_create_batch(const fl_lbm_params* params, int ninst, const size_t* offsets, const int32_t* h_x,
+                              const int32_t* h_y, const int32_t* h_z, const uint8_t* h_level,
+                              const uint8_t* h_state, fl_stream s, fl_lbm_solver** out) {
+    FL_REQUIRE(params && offsets && out, "null pointer");
     *out = nullptr;
     std::unique_ptr<fl_lbm_solver> S;
-    fl_status r = create_solver(p, h_x, h_y, h_z, h_level, h_state, n, FL_STREAM(s), S);
+    fl_status r = create_solver(params, ninst, offsets, h_x, h_y, h_z, h_level, h_state, FL_STREAM(s), S);
     if (r) return r;
     *out = S.release();
+    return FL_OK;
+}
+
+fl_status fl_lbm_plan_bytes(const fl_lbm_params* params, int ninst, const size_t* offsets, const int32_t* h_x,
+                            const int32_t* h_y, const int32_t* h_z, const uint8_t* h_level, const uint8_t* h_state,
+                            uint64_t* device_bytes) {
+    FL_REQUIRE(device_bytes, "null pointer");
+    std::unique_ptr<fl_lbm_solver> S;
+    fl_status r = plan_solver(params, ninst, offsets, h_x, h_y, h_z, h_level, h_state, S);
+    if (r) return r;
+    *device_bytes = layout_bytes(S.get());
     return FL_OK;
 }
 
 fl_status fl_lbm_destroy(fl_lbm_solver* solver) {
     if (!solver) return FL_OK;
     (void)hipStreamSynchronize(solver->stream);
-    if (solver->qslots) (void)hipFree(solver->qslots);
-    if (solver->qbuf) (void)hipFree(solver->qbuf);
     delete solver;
     return FL_OK;
 }
@@ -908,6 +1205,12 @@ fl_status fl_lbm_step(fl_lbm_solver* solver, int nsteps) {
     FL_REQUIRE(solver, "null solver");
     FL_REQUIRE(nsteps >= 0, "nsteps must be >= 0");
     return dispatch_step(solver, nsteps);
+}
+
+fl_status fl_lbm_num_instances(const fl_lbm_solver* S, int* n) {
+    FL_REQUIRE(S && n, "null pointer");
+    *n = S->ninst();
+    return FL_OK;
 }
 
 fl_status fl_lbm_stats_get(const fl_lbm_solver* S, fl_lbm_stats* out) {
@@ -923,36 +1226,64 @@ fl_status fl_lbm_stats_get(const fl_lbm_solver* S, fl_lbm_stats* out) {
         out->f2c[l] = H.f2c_ghost.size();
         out->capture[l] = H.cap_slots.size();
         out->updates_per_step += H.nfluid << l;
+        out->variant[l] = S->variant[l];
     }
     out->device_bytes = S->device_bytes;
+    out->ninst = S->ninst();
+    out->graph = S->use_graph ? 1 : 0;
+    out->graph_launches = S->graph_launches;
     return FL_OK;
+}
+
+fl_status fl_lbm_inst_leaves(const fl_lbm_solver* S, int inst, uint64_t* leaves, uint64_t* updates_per_step) {
+    FL_SOLVER_INST(S, inst);
+    FL_REQUIRE(leaves && updates_per_step, "null pointer");
+    *updates_per_step = 0;
+    for (int l = 0; l < 24; ++l) leaves[l] = 0;
+    for (int l = 0; l < S->L; ++l) {
+        leaves[l] = S->h[l].nfluid_inst[inst];
+        *updates_per_step += leaves[l] << l;
+    }
+    return FL_OK;
+}
+
+fl_status fl_lbm_get_f_inst(fl_lbm_solver* S, int inst, const int32_t* x, const int32_t* y, const int32_t* z,
+                            const uint8_t* lv, size_t n, double* f) {
+    FL_SOLVER_INST(S, inst);
+    FL_REQUIRE(n == 0 || (x && y && z && lv && f), "null pointer");
+    switch (S->storage) {
+        case FL_STORE_F32: return get_f<float>(S, inst, x, y, z, lv, n, f);
+        case FL_STORE_F16: return get_f<_Float16>(S, inst, x, y, z, lv, n, f);
+        default: return get_f<double>(S, inst, x, y, z, lv, n, f);
+    }
 }
 
 fl_status fl_lbm_get_f(fl_lbm_solver* S, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv,
                        size_t n, double* f) {
-    FL_REQUIRE(S && (n == 0 || (x && y && z && lv && f)), "null pointer");
-    switch (S->p.storage) {
-        case FL_STORE_F32: return get_f<float>(S, x, y, z, lv, n, f);
-        case FL_STORE_F16: return get_f<_Float16>(S, x, y, z, lv, n, f);
-        default: return get_f<double>(S, x, y, z, lv, n, f);
+    return fl_lbm_get_f_inst(S, 0, x, y, z, lv, n, f);
+}
+
+fl_status fl_lbm_set_f_inst(fl_lbm_solver* S, int inst, const int32_t* x, const int32_t* y, const int32_t* z,
+                            const uint8_t* lv, size_t n, const double* f) {
+    FL_SOLVER_INST(S, inst);
+    FL_REQUIRE(n == 0 || (x && y && z && lv && f), "null pointer");
+    switch (S->storage) {
+        case FL_STORE_F32: return set_f<float>(S, inst, x, y, z, lv, n, f);
+        case FL_STORE_F16: return set_f<_Float16>(S, inst, x, y, z, lv, n, f);
+        default: return set_f<double>(S, inst, x, y, z, lv, n, f);
     }
 }
 
 fl_status fl_lbm_set_f(fl_lbm_solver* S, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv,
                        size_t n, const double* f) {
-    FL_REQUIRE(S && (n == 0 || (x && y && z && lv && f)), "null pointer");
-    switch (S->p.storage) {
-        case FL_STORE_F32: return set_f<float>(S, x, y, z, lv, n, f);
-        case FL_STORE_F16: return set_f<_Float16>(S, x, y, z, lv, n, f);
-        default: return set_f<double>(S, x, y, z, lv, n, f);
-    }
+    return fl_lbm_set_f_inst(S, 0, x, y, z, lv, n, f);
 }
 
-fl_status fl_lbm_moments(fl_lbm_solver* S, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv,
-                         size_t n, double* rho, double* u) {
+fl_status fl_lbm_moments_inst(fl_lbm_solver* S, int inst, const int32_t* x, const int32_t* y, const int32_t* z,
+                              const uint8_t* lv, size_t n, double* rho, double* u) {
     FL_REQUIRE(S && (n == 0 || (rho && u)), "null pointer");
     std::vector<double> f(n * Q);
-    fl_status r = fl_lbm_get_f(S, x, y, z, lv, n, f.data());
+    fl_status r = fl_lbm_get_f_inst(S, inst, x, y, z, lv, n, f.data());
     if (r) return r;
     for (size_t k = 0; k < n; ++k) {
         double rr = 0, jx = 0, jy = 0, jz = 0;
@@ -971,91 +1302,203 @@ fl_status fl_lbm_moments(fl_lbm_solver* S, const int32_t* x, const int32_t* y, c
     return FL_OK;
 }
 
-fl_status fl_lbm_totals(fl_lbm_solver* S, double* mass, double* mom) {
-    FL_REQUIRE(S && mass && mom, "null pointer");
-    switch (S->p.storage) {
-        case FL_STORE_F32: return totals<float>(S, mass, mom);
-        case FL_STORE_F16: return totals<_Float16>(S, mass, mom);
-        default: return totals<double>(S, mass, mom);
-    }
+fl_status fl_lbm_moments(fl_lbm_solver* S, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv,
+                         size_t n, double* rho, double* u) {
+    return fl_lbm_moments_inst(S, 0, x, y, z, lv, n, rho, u);
 }
 
-fl_status fl_lbm_force(fl_lbm_solver* S, double* F) {
+fl_status fl_lbm_totals_batch(fl_lbm_solver* S, double* mass, double* mom) {
+    FL_REQUIRE(S && mass && mom, "null pointer");
+    std::vector<double> a((size_t)S->ninst() * 4);
+    fl_status r;
+    switch (S->storage) {
+        case FL_STORE_F32: r = totals<float>(S, a.data()); break;
+        case FL_STORE_F16: r = totals<_Float16>(S, a.data()); break;
+        default: r = totals<double>(S, a.data()); break;
+    }
+    if (r) return r;
+    for (int k = 0; k < S->ninst(); ++k) {
+        mass[k] = a[4 * k];
+        for (int c = 0; c < 3; ++c) mom[3 * k + c] = a[4 * k + 1 + c];
+    }
+    return FL_OK;
+}
+
+fl_status fl_lbm_totals(fl_lbm_solver* S, double* mass, double* mom) {
+    FL_REQUIRE(S && mass && mom, "null pointer");
+    std::vector<double> m(S->ninst()), j((size_t)S->ninst() * 3);
+    fl_status r = fl_lbm_totals_batch(S, m.data(), j.data());
+    if (r) return r;
+    *mass = 0;
+    mom[0] = mom[1] = mom[2] = 0;
+    for (int k = 0; k < S->ninst(); ++k) {
+        *mass += m[k];
+        for (int c = 0; c < 3; ++c) mom[c] += j[3 * k + c];
+    }
+    return FL_OK;
+}
+
+fl_status fl_lbm_force_batch(fl_lbm_solver* S, double* F) {
     FL_REQUIRE(S && F, "null pointer");
-    switch (S->p.storage) {
+    switch (S->storage) {
         case FL_STORE_F32: return force<float>(S, F);
         case FL_STORE_F16: return force<_Float16>(S, F);
         default: return force<double>(S, F);
     }
 }
 
+fl_status fl_lbm_force(fl_lbm_solver* S, double* F) {
+    FL_REQUIRE(S && F, "null pointer");
+    std::vector<double> a((size_t)S->ninst() * 3);
+    fl_status r = fl_lbm_force_batch(S, a.data());
+    if (r) return r;
+    F[0] = F[1] = F[2] = 0;
+    for (int k = 0; k < S->ninst(); ++k)
+        for (int c = 0; c < 3; ++c) F[c] += a[3 * k + c];
+    return FL_OK;
+}
+
+fl_status fl_lbm_velocity_change(fl_lbm_solver* S, double* du) {
+    FL_REQUIRE(S && du, "null pointer");
+    switch (S->storage) {
+        case FL_STORE_F32: return velocity_change<float>(S, du);
+        case FL_STORE_F16: return velocity_change<_Float16>(S, du);
+        default: return velocity_change<double>(S, du);
+    }
+}
+
+fl_status fl_lbm_set_live(fl_lbm_solver* S, int inst, int live) {
+    FL_SOLVER_INST(S, inst);
+    S->hinst[inst].live = live ? 1 : 0;
+    FL_HIP(hipMemcpyAsync(&S->dinst[inst].live, &S->hinst[inst].live, sizeof(int32_t), hipMemcpyHostToDevice,
+                          S->stream));
+    FL_HIP(hipStreamSynchronize(S->stream));
+    return FL_OK;
+}
+
+fl_status fl_lbm_set_graph(fl_lbm_solver* S, int enable) {
+    FL_REQUIRE(S, "null solver");
+    S->use_graph = enable != 0;
+    if (!S->use_graph) S->drop_graphs();
+    return FL_OK;
+}
+
+int fl_lbm_num_variants(void) { return collide_variants(); }
+
+const char* fl_lbm_variant_name(int v) { return collide_variant_name(v); }
+
+fl_status fl_lbm_set_variant(fl_lbm_solver* S, int level, int variant) {
+    FL_REQUIRE(S, "null solver");
+    FL_REQUIRE(level >= 0 && level < S->L, "level out of range");
+    FL_REQUIRE(variant >= 0 && variant < collide_variants(), "unknown collide variant");
+    S->variant[level] = variant;
+    S->drop_graphs();
+    return FL_OK;
+}
+
+fl_status fl_lbm_time_collide_variant(fl_lbm_solver* S, int level, int variant, int iters, float* ms_per_launch) {
+    FL_REQUIRE(S && ms_per_launch, "null pointer");
+    FL_REQUIRE(level >= 0 && level < S->L && iters > 0, "bad level or iteration count");
+    FL_REQUIRE(variant >= 0 && variant < collide_variants(), "unknown collide variant");
+    return time_any(S, level, variant, iters, ms_per_launch);
+}
+
 fl_status fl_lbm_time_collide(fl_lbm_solver* S, int level, int iters, float* ms_per_launch) {
     FL_REQUIRE(S && ms_per_launch, "null pointer");
     FL_REQUIRE(level >= 0 && level < S->L && iters > 0, "bad level or iteration count");
-    DevLevel& D = S->d[level];
-    hipEvent_t a, b;
-    FL_HIP(hipEventCreate(&a));
-    FL_HIP(hipEventCreate(&b));
-    FL_HIP(hipEventRecord(a, S->stream));
-    for (int it = 0; it < iters; ++it) {
-        hipError_t e;
-        switch (S->p.storage) {
-            case FL_STORE_F32:
-                e = launch_collide<float>(D.ld, S->bc, static_cast<const float*>(D.buf[D.cur]),
-                                          static_cast<float*>(D.buf[1 - D.cur]), 1.0 / D.tau, S->stream);
-                break;
-            case FL_STORE_F16:
-                e = launch_collide<_Float16>(D.ld, S->bc, static_cast<const _Float16*>(D.buf[D.cur]),
-                                             static_cast<_Float16*>(D.buf[1 - D.cur]), 1.0 / D.tau, S->stream);
-                break;
-            default:
-                e = launch_collide<double>(D.ld, S->bc, static_cast<const double*>(D.buf[D.cur]),
-                                           static_cast<double*>(D.buf[1 - D.cur]), 1.0 / D.tau, S->stream);
-                break;
+    return time_any(S, level, S->variant[level], iters, ms_per_launch);
+}
+
+fl_status fl_lbm_autotune(fl_lbm_solver* S, int iters, int* chosen) {
+    FL_REQUIRE(S, "null solver");
+    FL_REQUIRE(iters > 0, "iters must be positive");
+    for (int l = 0; l < S->L; ++l) {
+        if (S->h[l].nactive == 0) {
+            if (chosen) chosen[l] = S->variant[l];
+            continue;
         }
-        FL_HIP(e);
-        D.cur ^= 1;
+        // an even number of launches per variant leaves the level's state
+        // parity unchanged; one warm-up pair per variant
+        const int it2 = iters + (iters & 1);
+        int best = S->variant[l];
+        float best_ms = 1e30f;
+        for (int v = 0; v < collide_variants(); ++v) {
+            float ms;
+            fl_status r = time_any(S, l, v, 2, &ms);
+            if (!r) r = time_any(S, l, v, it2, &ms);
+            if (r) return r;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = v;
+            }
+        }
+        S->variant[l] = best;
+        if (chosen) chosen[l] = best;
     }
-    FL_HIP(hipEventRecord(b, S->stream));
-    FL_HIP(hipEventSynchronize(b));
-    float ms = 0;
-    FL_HIP(hipEventElapsedTime(&ms, a, b));
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    *ms_per_launch = ms / iters;
+    S->drop_graphs();
     return FL_OK;
+}
+
+fl_status fl_lbm_vorticity_inst(fl_lbm_solver* S, int inst, const int32_t* x, const int32_t* y, const int32_t* z,
+                                const uint8_t* lv, size_t n, double* out) {
+    FL_SOLVER_INST(S, inst);
+    FL_REQUIRE(n == 0 || (x && y && z && lv && out), "null pointer");
+    switch (S->storage) {
+        case FL_STORE_F32: return vorticity<float>(S, inst, x, y, z, lv, n, out);
+        case FL_STORE_F16: return vorticity<_Float16>(S, inst, x, y, z, lv, n, out);
+        default: return vorticity<double>(S, inst, x, y, z, lv, n, out);
+    }
 }
 
 fl_status fl_lbm_vorticity(fl_lbm_solver* S, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv,
                            size_t n, double* out) {
-    FL_REQUIRE(S && (n == 0 || (x && y && z && lv && out)), "null pointer");
-    switch (S->p.storage) {
-        case FL_STORE_F32: return vorticity<float>(S, x, y, z, lv, n, out);
-        case FL_STORE_F16: return vorticity<_Float16>(S, x, y, z, lv, n, out);
-        default: return vorticity<double>(S, x, y, z, lv, n, out);
-    }
+    return fl_lbm_vorticity_inst(S, 0, x, y, z, lv, n, out);
 }
 
-fl_status fl_lbm_regrid(fl_lbm_solver* S, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv,
-                        const uint8_t* state, size_t n) {
-    FL_REQUIRE(S, "null solver");
+fl_status fl_lbm_regrid_batch(fl_lbm_solver* S, const size_t* offsets, const int32_t* x, const int32_t* y,
+                              const int32_t* z, const uint8_t* lv, const uint8_t* state) {
+    FL_REQUIRE(S && offsets, "null pointer");
+    std::vector<fl_lbm_params> ps;
+    for (const auto& ih : S->inst) ps.push_back(ih.p);
     std::unique_ptr<fl_lbm_solver> N;
-    fl_lbm_params p = S->p;
-    fl_status r = create_solver(&p, x, y, z, lv, state, n, S->stream, N);
+    fl_status r = plan_solver(ps.data(), S->ninst(), offsets, x, y, z, lv, state, N);
     if (r) return r;
-    switch (S->p.storage) {
+    N->stream = S->stream;
+    N->hinst = S->hinst;  // keeps the live flags
+    switch (S->storage) {
+        case FL_STORE_F32: r = upload_and_init<float>(N.get()); break;
+        case FL_STORE_F16: r = upload_and_init<_Float16>(N.get()); break;
+        default: r = upload_and_init<double>(N.get()); break;
+    }
+    if (r) return r;
+    switch (S->storage) {
         case FL_STORE_F32: r = remap<float>(S, N.get()); break;
         case FL_STORE_F16: r = remap<_Float16>(S, N.get()); break;
         default: r = remap<double>(S, N.get()); break;
     }
     if (r) return r;
-    // the solver handle keeps its identity; its contents become the new mesh
-    std::swap(S->h, N->h);
-    std::swap(S->d, N->d);
+    // the handle keeps its identity, stream, variants and graph setting; its
+    // layout becomes the new mesh (the old one is freed with N)
+    S->drop_graphs();
+    std::swap(S->inst, N->inst);
+    std::swap(S->L, N->L);
+    for (int l = 0; l < 24; ++l) {
+        std::swap(S->h[l], N->h[l]);
+        std::swap(S->d[l], N->d[l]);
+    }
+    std::swap(S->dinst, N->dinst);
+    std::swap(S->acc, N->acc);
     std::swap(S->allocs, N->allocs);
-    std::swap(S->partial, N->partial);
     std::swap(S->device_bytes, N->device_bytes);
     return FL_OK;
+}
+
+fl_status fl_lbm_regrid(fl_lbm_solver* S, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv,
+                        const uint8_t* state, size_t n) {
+    FL_REQUIRE(S, "null solver");
+    FL_REQUIRE(S->ninst() == 1, "fl_lbm_regrid is for single-instance solvers; use fl_lbm_regrid_batch");
+    const size_t off[2] = {0, n};
+    return fl_lbm_regrid_batch(S, off, x, y, z, lv, state);
 }
 
 fl_status fl_lbm_stream(fl_lbm_solver* S, fl_stream* s) {

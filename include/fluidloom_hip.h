@@ -255,18 +255,56 @@ typedef struct {
     int32_t pad;
     uint64_t updates_per_step;   /* sum over levels of leaves * 2^level */
     uint64_t device_bytes;
+    int32_t variant[24];    /* collide-stream variant per level (fl_lbm_variant_name) */
+    int32_t ninst;          /* instances in the batch */
+    int32_t graph;          /* 1 when steps replay a captured hipGraph */
+    uint64_t graph_launches;
 } fl_lbm_stats;
 
 typedef struct fl_lbm_solver fl_lbm_solver;
 
-/* Leaves are host arrays (native coords, state 0 fluid / 1 solid). */
+/* Leaves are host arrays (native coords, state 0 fluid / 1 solid).  A NULL
+ * stream makes the solver create (and own) a stream of its own, so that its
+ * step can be graph-captured.  Single-instance form of fl_lbm_create_batch. */
 fl_status fl_lbm_create(const fl_lbm_params* p, const int32_t* h_x, const int32_t* h_y,
                         const int32_t* h_z, const uint8_t* h_level, const uint8_t* h_state,
                         size_t n, fl_stream s, fl_lbm_solver** out);
+/* A batch of independent instances in one layout: instance k owns leaves
+ * [offsets[k], offsets[k+1]) of the concatenated arrays and its own
+ * params[k] (grid, tau, boundaries, levels); every instance uses the same
+ * storage.  Bricks of all instances are concatenated per level (grouped by
+ * instance, Hilbert order inside), so one launch per level steps the whole
+ * batch.  Results of an instance do not depend on the rest of the batch. */
+fl_status fl_lbm_create_batch(const fl_lbm_params* params, int ninst, const size_t* offsets,
+                              const int32_t* h_x, const int32_t* h_y, const int32_t* h_z,
+                              const uint8_t* h_level, const uint8_t* h_state, fl_stream s,
+                              fl_lbm_solver** out);
+/* Device bytes fl_lbm_create_batch would allocate for these leaves (host
+ * layout only; no device work) -- the admission footprint of a request. */
+fl_status fl_lbm_plan_bytes(const fl_lbm_params* params, int ninst, const size_t* offsets,
+                            const int32_t* h_x, const int32_t* h_y, const int32_t* h_z,
+                            const uint8_t* h_level, const uint8_t* h_state, uint64_t* device_bytes);
+fl_status fl_lbm_num_instances(const fl_lbm_solver* solver, int* n);
+/* fluid leaves per level of one instance and its updates per coarse step */
+fl_status fl_lbm_inst_leaves(const fl_lbm_solver* solver, int inst, uint64_t* h_leaves24,
+                             uint64_t* h_updates_per_step);
 fl_status fl_lbm_destroy(fl_lbm_solver* solver);
 fl_status fl_lbm_step(fl_lbm_solver* solver, int nsteps);
 fl_status fl_lbm_stats_get(const fl_lbm_solver* solver, fl_lbm_stats* out);
-/* host-side queries at given leaves (synchronous) */
+/* host-side queries at given leaves (synchronous); the forms without _inst
+ * address instance 0 */
+fl_status fl_lbm_get_f_inst(fl_lbm_solver* solver, int inst, const int32_t* h_x,
+                            const int32_t* h_y, const int32_t* h_z, const uint8_t* h_level,
+                            size_t n, double* h_f);
+fl_status fl_lbm_set_f_inst(fl_lbm_solver* solver, int inst, const int32_t* h_x,
+                            const int32_t* h_y, const int32_t* h_z, const uint8_t* h_level,
+                            size_t n, const double* h_f);
+fl_status fl_lbm_moments_inst(fl_lbm_solver* solver, int inst, const int32_t* h_x,
+                              const int32_t* h_y, const int32_t* h_z, const uint8_t* h_level,
+                              size_t n, double* h_rho, double* h_u);
+fl_status fl_lbm_vorticity_inst(fl_lbm_solver* solver, int inst, const int32_t* h_x,
+                                const int32_t* h_y, const int32_t* h_z, const uint8_t* h_level,
+                                size_t n, double* h_out);
 fl_status fl_lbm_get_f(fl_lbm_solver* solver, const int32_t* h_x, const int32_t* h_y,
                        const int32_t* h_z, const uint8_t* h_level, size_t n, double* h_f);
 fl_status fl_lbm_set_f(fl_lbm_solver* solver, const int32_t* h_x, const int32_t* h_y,
@@ -287,10 +325,38 @@ fl_status fl_lbm_vorticity(fl_lbm_solver* solver, const int32_t* h_x, const int3
  * (AdaptationEngine.cpp:540-549).  Rebuilds the solver's device layout. */
 fl_status fl_lbm_regrid(fl_lbm_solver* solver, const int32_t* h_x, const int32_t* h_y,
                         const int32_t* h_z, const uint8_t* h_level, const uint8_t* h_state, size_t n);
-/* FP64 device reductions (synchronous) */
+/* Batch form: instance k's leaves are [offsets[k], offsets[k+1]).  Live
+ * flags, collide variants and the graph setting are kept. */
+fl_status fl_lbm_regrid_batch(fl_lbm_solver* solver, const size_t* offsets, const int32_t* h_x,
+                              const int32_t* h_y, const int32_t* h_z, const uint8_t* h_level,
+                              const uint8_t* h_state);
+/* FP64 device reductions (synchronous).  Per instance: per-brick wave sums,
+ * then one workgroup per instance over its brick range in a fixed order.
+ * fl_lbm_totals / fl_lbm_force sum the instances. */
 fl_status fl_lbm_totals(fl_lbm_solver* solver, double* h_mass, double* h_mom3);
 fl_status fl_lbm_force(fl_lbm_solver* solver, double* h_force3);
+fl_status fl_lbm_totals_batch(fl_lbm_solver* solver, double* h_mass, double* h_mom3n);
+fl_status fl_lbm_force_batch(fl_lbm_solver* solver, double* h_force3n);
+/* Per instance, max over its fluid leaves of |u - u_snapshot| (infinity
+ * norm over components), then snapshot = u.  The first call takes the
+ * snapshot and reports HUGE_VAL.  Steady-state test of the serving loop. */
+fl_status fl_lbm_velocity_change(fl_lbm_solver* solver, double* h_du);
+/* A non-live instance is no longer stepped (its slots keep their values);
+ * the serving loop retires finished requests this way. */
+fl_status fl_lbm_set_live(fl_lbm_solver* solver, int inst, int live);
+/* enable != 0: fl_lbm_step replays a hipGraph of two captured steps
+ * (captured on first use, re-captured after a regrid or variant change). */
+fl_status fl_lbm_set_graph(fl_lbm_solver* solver, int enable);
 fl_status fl_lbm_stream(fl_lbm_solver* solver, fl_stream* s);
+/* Collide-stream variants (bricks per workgroup x launch shape) and the
+ * autotuner that times each on the live layout and keeps the fastest per
+ * level (h_chosen: one entry per level, may be NULL). */
+int fl_lbm_num_variants(void);
+const char* fl_lbm_variant_name(int variant);
+fl_status fl_lbm_set_variant(fl_lbm_solver* solver, int level, int variant);
+fl_status fl_lbm_autotune(fl_lbm_solver* solver, int iters, int* h_chosen);
+fl_status fl_lbm_time_collide_variant(fl_lbm_solver* solver, int level, int variant, int iters,
+                                      float* ms_per_launch);
 /* Measurement: `iters` back-to-back collide-stream launches of one level on
  * the solver's stream, timed with HIP events (advances that level's state;
  * a measurement of the kernel alone, not part of a physical step). */

@@ -35,3 +35,38 @@ def test_error_convention_without_device():
     assert st == _lib.FL_E_INVALID
     assert "null" in L.fl_last_error_string().decode()
     assert L.fl_hash_capacity(700000) == 1 << 21
+
+
+def test_plan_bytes_host_only(oracle):
+    """fl_lbm_plan_bytes runs the layout on the host (no GPU needed) and
+    reports the footprint a batch would allocate; mesh errors surface here."""
+    from fluidloom_amd import _lib
+    from fluidloom_amd.solver import make_params, plan_bytes
+    from tests.lbm_cases import case
+    cs = [case(n, oracle) for n in ("cavity", "cavity2", "sphere3")]
+
+    def pm(c):
+        nx, ny, nz = c["n0"]
+        return make_params(nx, ny, nz, c["max_level"], c["tau0"], c["bc"], c["bc_u"], "f32", u_init=c["u_init"])
+
+    def mesh(c):
+        cc = c["cells"]
+        return dict(x=cc.x, y=cc.y, z=cc.z, level=cc.level, state=cc.state)
+
+    one = [plan_bytes([pm(c)], [mesh(c)]) for c in cs]
+    both = plan_bytes([pm(c) for c in cs], [mesh(c) for c in cs])
+    assert all(b > 0 for b in one)
+    # per-instance tables add a little; the bulk is additive
+    assert abs(both - sum(one)) < 0.01 * sum(one)
+    p16 = make_params(8, 8, 8, 0, 0.6, ["wall"] * 6, [(0, 0, 0)] * 6, "f16")
+    p32 = make_params(8, 8, 8, 0, 0.6, ["wall"] * 6, [(0, 0, 0)] * 6, "f32")
+    m = mesh(cs[0])
+    assert plan_bytes([p16], [m]) < plan_bytes([p32], [m])
+    # leaves above the declared max_level
+    c = cs[1]["cells"]
+    lv = c.level.copy()
+    bad = dict(x=c.x, y=c.y, z=c.z, level=lv, state=c.state)
+    p = pm(cs[1])
+    p.max_level = 0
+    with pytest.raises(_lib.FluidLoomError):
+        plan_bytes([p], [bad])
