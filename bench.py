@@ -72,6 +72,25 @@ def cpu_baseline(budget_s=15.0):
                       f"{steps} steps in {dt:.1f} s"}
 
 
+def copy_bandwidth(_lib, L, stream, nbytes=2 << 30, iters=10):
+    C = _lib.C
+    a, b = _lib.DeviceArray(nbytes, np.uint8), _lib.DeviceArray(nbytes, np.uint8)
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    _lib.check(L.fl_event_create(C.byref(e0)))
+    _lib.check(L.fl_event_create(C.byref(e1)))
+    _lib.check(L.fl_copy_kernel(b.ptr, a.ptr, nbytes, stream))
+    _lib.check(L.fl_event_record(e0, stream))
+    for _ in range(iters):
+        _lib.check(L.fl_copy_kernel(b.ptr, a.ptr, nbytes, stream))
+    _lib.check(L.fl_event_record(e1, stream))
+    _lib.check(L.fl_stream_sync(stream))
+    ms = C.c_float(0)
+    _lib.check(L.fl_event_elapsed_ms(e0, e1, C.byref(ms)))
+    a.free()
+    b.free()
+    return 2 * nbytes * iters / (ms.value * 1e-3) / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +102,7 @@ def main():
     ap.add_argument("--regrid-every", type=int, default=100)
     ap.add_argument("--no-graph", action="store_true", help="step eagerly instead of replaying a hipGraph")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default collide variant")
+    ap.add_argument("--variant", default=None, help="force this collide variant on every level (no autotune)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
@@ -114,7 +134,10 @@ def main():
         f"{[int(st.bricks[l]) for l in range(levels + 1)]}, {upd} updates/step, "
         f"{st.device_bytes / 2**30:.2f} GiB")
     from fluidloom_amd.solver import variant_names
-    if not args.no_autotune:
+    if args.variant:
+        for l in range(levels + 1):
+            solver.set_variant(l, variant_names().index(args.variant))
+    elif not args.no_autotune:
         chosen = solver.autotune(6)
         log(f"autotuned collide variants per level: {[variant_names()[v] for v in chosen]}")
     if not args.no_graph:
@@ -171,6 +194,10 @@ def main():
     achieved = alg_bytes / (ms_k * 1e-3) / 1e9
     log(f"level-{levels} collide: {ms_k:.3f} ms/launch, {achieved:.0f} GB/s algorithmic")
 
+    # bandwidth reference: a plain 16 B/lane copy kernel over 2 x 2 GiB
+    copy_gbs = copy_bandwidth(_lib, L, stream)
+    log(f"plain copy kernel: {copy_gbs:.0f} GB/s")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, bounded sample)...")
@@ -204,6 +231,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": f"k_collide level {levels} ({leaves[levels]} leaves x {bpu} B)"},
+        "copy_kernel_gbs": round(copy_gbs, 1),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

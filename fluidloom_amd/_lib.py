@@ -91,6 +91,7 @@ _SIGS = {
     "fl_event_record": (C.c_int, [vp, vp]),
     "fl_event_elapsed_ms": (C.c_int, [vp, vp, C.POINTER(C.c_float)]),
     "fl_mem_info": (C.c_int, [C.POINTER(sz), C.POINTER(sz)]),
+    "fl_copy_kernel": (C.c_int, [vp, vp, sz, vp]),
     "fl_hilbert_encode": (C.c_int, [vp, vp, vp, vp, C.c_int, vp, sz, vp]),
     "fl_hilbert_decode": (C.c_int, [vp, vp, C.c_int, vp, vp, vp, sz, vp]),
     "fl_hilbert_encode_aos4": (C.c_int, [vp, C.c_int, vp, sz, vp]),

@@ -21,6 +21,8 @@
 
 namespace fl {
 fl_status hash_clear_impl(uint64_t* tk, uint32_t* tv, uint64_t cap, hipStream_t s);
+fl_status cellmap_build_impl(uint64_t* tk, uint32_t* tv, uint64_t cap, const uint64_t* keys, size_t n,
+                             hipStream_t s);
 fl_status hash_build_impl(uint64_t* tk, uint32_t* tv, uint64_t cap, const uint64_t* keys,
                           const uint32_t* vals, size_t n, uint32_t* probes, hipStream_t s);
 
@@ -65,7 +67,7 @@ __device__ inline uint32_t find_cell(const CellHash& h, int l, int32_t x, int32_
     if (l < 0 || l > 19 || x < 0 || y < 0 || z < 0 || x >= kKeyCoordLimit || y >= kKeyCoordLimit ||
         z >= kKeyCoordLimit)
         return kInvalid;
-    return hash_lookup(h.tk, h.tv, h.cap, cell_hkey((uint32_t)x, (uint32_t)y, (uint32_t)z, l));
+    return cell_lookup(h.tk, h.tv, h.cap, cell_hkey((uint32_t)x, (uint32_t)y, (uint32_t)z, l));
 }
 
 struct Root {
@@ -107,7 +109,7 @@ fl_status build_cell_hash(CellsC c, size_t n, uint64_t* keys, uint64_t* tk, uint
     FL_CHECK_LAUNCH();
     fl_status st = hash_clear_impl(tk, tv, cap, s);
     if (st) return st;
-    return hash_build_impl(tk, tv, cap, keys, nullptr, n, nullptr, s);
+    return cellmap_build_impl(tk, tv, cap, keys, n, s);
 }
 
 // ---------------------------------------------------------------- a9
@@ -187,7 +189,7 @@ __global__ __launch_bounds__(kBlock) void k_detect0(CellsC c, size_t n, int M,
                     if (qx < 0 || qy < 0 || qz < 0 || qx >= (1 << M) || qy >= (1 << M) || qz >= (1 << M))
                         continue;
                     uint64_t key = cell_hkey((uint32_t)qx, (uint32_t)qy, (uint32_t)qz, l);
-                    if (hash_lookup(h.tk, h.tv, h.cap, key) != kInvalid) {
+                    if (cell_lookup(h.tk, h.tv, h.cap, key) != kInvalid) {
                         if (l > s + 1) v = 1;
                         break;
                     }
@@ -597,7 +599,7 @@ fl_status run_balance(const fl_regrid_config* cfg, fl_cells* cells, size_t n, co
             FL_CHECK_LAUNCH();
             fl_status st = hash_clear_impl(tk, tv, x.L.cap, x.s);
             if (st) return st;
-            st = hash_build_impl(tk, tv, x.L.cap, keys, nullptr, n, nullptr, x.s);
+            st = cellmap_build_impl(tk, tv, x.L.cap, keys, n, x.s);
             if (st) return st;
             hipLaunchKernelGGL(k_detect0, dim3(sgrid(n)), dim3(kBlock), 0, x.s, c, n, cfg->anchor_level, shadow,
                                h, viol, cnt);

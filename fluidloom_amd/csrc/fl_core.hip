@@ -21,6 +21,14 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line) {
 }
 }  // namespace fl
 
+namespace {
+// Plain streaming copy, 16 B per lane per iteration: the bandwidth
+// reference (and HBM-counter calibration) for the other kernels.
+__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+}  // namespace
+
 extern "C" {
 
 const char* fl_last_error_string(void) { return fl::g_last_error.c_str(); }
@@ -88,6 +96,19 @@ fl_status fl_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes, fl_stream 
 fl_status fl_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes, fl_stream s) {
     if (bytes == 0) return FL_OK;
     FL_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, FL_STREAM(s)));
+    return FL_OK;
+}
+
+fl_status fl_copy_kernel(void* d_dst, const void* d_src, size_t bytes, fl_stream s) {
+    FL_REQUIRE(bytes % 16 == 0, "fl_copy_kernel: bytes must be a multiple of 16");
+    FL_REQUIRE(((uintptr_t)d_dst | (uintptr_t)d_src) % 16 == 0, "fl_copy_kernel: pointers must be 16-byte aligned");
+    const size_t n = bytes / 16;
+    if (n == 0) return FL_OK;
+    size_t g = (n + 255) / 256;
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)g), dim3(256), 0, FL_STREAM(s), static_cast<const uint4*>(d_src),
+                       static_cast<uint4*>(d_dst), n);
+    FL_HIP(hipGetLastError());
     return FL_OK;
 }
 

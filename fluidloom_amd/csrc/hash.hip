@@ -40,6 +40,14 @@ __global__ __launch_bounds__(kBlock) void k_query(const uint64_t* __restrict__ t
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_cell_build(uint64_t* __restrict__ tk, uint32_t* __restrict__ tv,
+                                                       uint64_t cap, const uint64_t* __restrict__ keys, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const uint64_t k = keys[i];
+        if (k != kEmptyKey) cell_insert(tk, tv, cap, k, (uint32_t)i);
+    }
+}
+
 unsigned cap_grid(size_t n) {
     size_t g = (n + kBlock - 1) / kBlock;
     if (g > 8192) g = 8192;
@@ -59,6 +67,15 @@ fl_status hash_build_impl(uint64_t* tk, uint32_t* tv, uint64_t cap, const uint64
                           const uint32_t* vals, size_t n, uint32_t* probes, hipStream_t s) {
     if (n == 0) return FL_OK;
     hipLaunchKernelGGL(k_build, dim3(cap_grid(n)), dim3(kBlock), 0, s, tk, tv, cap, keys, vals, n, probes);
+    FL_CHECK_LAUNCH();
+    return FL_OK;
+}
+
+// exact cell map (value = smallest index holding the key) for the AMR kernels
+fl_status cellmap_build_impl(uint64_t* tk, uint32_t* tv, uint64_t cap, const uint64_t* keys, size_t n,
+                             hipStream_t s) {
+    if (n == 0) return FL_OK;
+    hipLaunchKernelGGL(k_cell_build, dim3(cap_grid(n)), dim3(kBlock), 0, s, tk, tv, cap, keys, n);
     FL_CHECK_LAUNCH();
     return FL_OK;
 }

@@ -58,4 +58,41 @@ __device__ inline uint32_t hash_lookup(const uint64_t* __restrict__ tkeys, const
     return 0xFFFFFFFFu;
 }
 
+// Exact map for the AMR cell lookups (not the reference's table): the
+// packed cell keys (L<<57 | x<<38 | y<<19 | z) carry z in their low bits, so
+// the slot is the top log2(cap) bits of the Fibonacci product instead of the
+// low ones, and probing runs until a hit or an empty slot.  The capacity
+// rule keeps the load <= 0.6, so an empty slot always exists and no key is
+// ever dropped.
+__device__ inline uint64_t cell_slot(uint64_t key, uint64_t cap) {
+    const int bits = 63 - __clzll((long long)cap);
+    return (key * 0x9e3779b97f4a7c15ull) >> (64 - bits);
+}
+
+__device__ inline void cell_insert(uint64_t* tkeys, uint32_t* tvals, uint64_t cap, uint64_t key, uint32_t val) {
+    uint64_t s = cell_slot(key, cap);
+    for (uint64_t p = 0; p < cap; ++p) {
+        unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&tkeys[s]),
+                                            0xFFFFFFFFFFFFFFFFull, (unsigned long long)key);
+        if (prev == 0xFFFFFFFFFFFFFFFFull || prev == key) {
+            atomicMin(&tvals[s], val);
+            return;
+        }
+        s = (s + 1) & (cap - 1);
+    }
+}
+
+__device__ inline uint32_t cell_lookup(const uint64_t* __restrict__ tkeys, const uint32_t* __restrict__ tvals,
+                                       uint64_t cap, uint64_t key) {
+    if (key == 0xFFFFFFFFFFFFFFFFull) return 0xFFFFFFFFu;
+    uint64_t s = cell_slot(key, cap);
+    for (uint64_t p = 0; p < cap; ++p) {
+        const uint64_t k = tkeys[s];
+        if (k == key) return tvals[s];
+        if (k == 0xFFFFFFFFFFFFFFFFull) return 0xFFFFFFFFu;
+        s = (s + 1) & (cap - 1);
+    }
+    return 0xFFFFFFFFu;
+}
+
 }  // namespace fl

@@ -637,7 +637,10 @@ template <typename T>
 fl_status advance(fl_lbm_solver* S, int l) {
     DevLevel& D = S->d[l];
     const HostLevel& H = S->h[l];
-    const bool fine = l + 1 < S->L && S->h[l + 1].nfluid > 0;
+    // levels are advanced while any deeper level holds fluid leaves (a mesh
+    // may leave coarse levels empty, e.g. a uniformly refined one)
+    bool fine = false;
+    for (int k = l + 1; k < S->L; ++k) fine = fine || S->h[k].nfluid > 0;
     hipStream_t st = S->stream;
     const uint32_t ncap = (uint32_t)H.cap_slots.size();
     if (fine) {
@@ -1369,6 +1372,24 @@ fl_status fl_lbm_velocity_change(fl_lbm_solver* S, double* du) {
 
 fl_status fl_lbm_set_live(fl_lbm_solver* S, int inst, int live) {
     FL_SOLVER_INST(S, inst);
+    if (S->hinst[inst].live && !live) {
+        // the instance's last state sits in the current buffer of each level;
+        // copy it into the other one so that it reads the same whatever the
+        // buffer parity after later steps
+        const size_t sb = store_bytes(S->storage);
+        for (int l = 0; l < S->L; ++l) {
+            const HostLevel& H = S->h[l];
+            DevLevel& D = S->d[l];
+            const size_t b0 = H.abeg[inst], b1 = H.abeg[inst + 1];
+            if (b1 == b0) continue;
+            for (int q = 0; q < Q; ++q) {
+                const size_t off = ((size_t)q * D.ld.S + b0 * 64) * sb;
+                FL_HIP(hipMemcpyAsync(static_cast<char*>(D.buf[1 - D.cur]) + off,
+                                      static_cast<const char*>(D.buf[D.cur]) + off, (b1 - b0) * 64 * sb,
+                                      hipMemcpyDeviceToDevice, S->stream));
+            }
+        }
+    }
     S->hinst[inst].live = live ? 1 : 0;
     FL_HIP(hipMemcpyAsync(&S->dinst[inst].live, &S->hinst[inst].live, sizeof(int32_t), hipMemcpyHostToDevice,
                           S->stream));

@@ -67,3 +67,16 @@ def sphere(nx0=256, ny0=128, nz0=128, center=(64.0, 64.0, 64.0), radius=16.0, re
 
 def cavity_base(n0=32):
     return uniform(n0, n0, n0)
+
+
+def cavity(n0, levels, log=None):
+    """Lid-driven cavity of n0^3 base cells with `levels` refinement levels
+    in a ball of radius 0.3 n0 centred on the middle of the lid (z = n0),
+    where the shear layer sits."""
+    base = uniform(n0, n0, n0)
+    if levels == 0:
+        return base
+    cells = refine_ball(base, (n0, n0, n0), np.array([n0 / 2, n0 / 2, float(n0)]), 0.3 * n0, levels, log=log)
+    h = cells.to_host()
+    cells.free()
+    return h

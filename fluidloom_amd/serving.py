@@ -1,0 +1,261 @@
+"""Serving with continuous batching.
+
+Requests (cavity and sphere solves of different sizes) arrive as a Poisson
+process.  At every scheduling tick the server
+
+  1. admits the requests that have arrived, in arrival order, while their
+     device footprint (fl_lbm_plan_bytes of their own layout) fits the
+     memory budget (free HBM at start-up, capped at 288 GB, times a safety
+     fraction); the admitted ones form a new cohort -- one LbmBatch whose
+     layout holds all of them, with its own stream and hipGraph;
+  2. enqueues K steps on every cohort (each on its own stream, so the
+     cohorts' work overlaps on the device);
+  3. checks each live request for steady state (max |u(t) - u(t-K)| / K
+     below the tolerance) or its step limit; a finished request is retired
+     (fl_lbm_set_live 0: its bricks are skipped from then on) and its
+     time-to-solution recorded;
+  4. frees a cohort once none of its requests is live.
+
+A request's trajectory does not depend on what it is batched with (bricks of
+different instances never interact, reductions are per instance), so its
+result and its step count are the same alone and in the mix.
+
+    python -m fluidloom_amd.serving --rate 20 --duration 600
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import meshes
+from ._lib import check, lib
+from .solver import LbmBatch, make_params, plan_bytes
+
+HBM_CAP = 288 * 10 ** 9
+
+
+def log(msg):
+    print(f"[serving] {msg}", file=sys.stderr, flush=True)
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float
+    kind: str
+    spec: dict
+    params: object = None
+    mesh: dict = None
+    footprint: int = 0
+    admitted: float = -1.0
+    finished: float = -1.0
+    steps: int = 0
+    updates_per_step: int = 0
+    converged: bool = False
+    result: dict = field(default_factory=dict)
+
+
+def request_specs(rate, duration, seed=42, scale=1, levels_max=3):
+    """Poisson arrivals (rate per second over `duration` seconds): 70 %
+    cavities, base n0 log-uniform in 16..128; 30 % spheres, base from
+    128x64x64 to 256x128x128, radius U(8, 16), Re U(50, 200), inlet 0.05.
+    1..levels_max grid levels.  Sizes and radii are divided by `scale`
+    (bricks need multiples of 4).  Generator: numpy MT19937(seed)."""
+    rng = np.random.Generator(np.random.MT19937(seed))
+    t, rid, out = 0.0, 0, []
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t >= duration:
+            break
+        nlev = int(rng.integers(1, levels_max + 1))
+        if rng.uniform() < 0.7:
+            n0 = float(np.exp(rng.uniform(np.log(16), np.log(128)))) / scale
+            n0 = int(max(8, 4 * round(n0 / 4)))
+            spec = dict(n0=n0, lid=float(rng.uniform(0.02, 0.10)), tau=float(rng.uniform(0.55, 0.9)),
+                        max_level=nlev - 1)
+            out.append(Request(rid, t, "cavity", spec))
+        else:
+            f = float(rng.uniform(1.0, 2.0))
+            nx = int(max(16, 4 * round(128 * f / scale / 4)))
+            r = float(rng.uniform(8.0, 16.0)) / scale
+            re = float(rng.uniform(50.0, 200.0))
+            spec = dict(nx=nx, ny=nx // 2, nz=nx // 2, r=r, re=re, u_in=0.05, max_level=nlev - 1)
+            out.append(Request(rid, t, "sphere", spec))
+        rid += 1
+    return out
+
+
+def prepare(req: Request, storage):
+    """Mesh (device regrid), lattice parameters and device footprint."""
+    s = req.spec
+    if req.kind == "cavity":
+        n0 = s["n0"]
+        req.mesh = meshes.cavity(n0, s["max_level"])
+        bcu = [(0.0, 0.0, 0.0)] * 5 + [(s["lid"], 0.0, 0.0)]
+        req.params = make_params(n0, n0, n0, s["max_level"], s["tau"], ["wall"] * 6, bcu, storage)
+    else:
+        nx, ny, nz, r = s["nx"], s["ny"], s["nz"], s["r"]
+        ctr = (nx / 4.0, ny / 2.0, nz / 2.0)
+        req.mesh = meshes.sphere(nx, ny, nz, ctr, r, 2 * r, s["max_level"])
+        tau0 = 3 * s["u_in"] * 2 * r / s["re"] + 0.5
+        bc = ["wall", "outflow", "wall", "wall", "wall", "wall"]
+        bcu = [(s["u_in"], 0.0, 0.0)] + [(0.0, 0.0, 0.0)] * 5
+        req.params = make_params(nx, ny, nz, s["max_level"], tau0, bc, bcu, storage, u_init=(s["u_in"], 0.0, 0.0))
+    req.footprint = plan_bytes([req.params], [req.mesh])
+
+
+class Cohort:
+    def __init__(self, reqs, graph=True):
+        self.reqs = reqs
+        self.batch = LbmBatch([r.params for r in reqs], [r.mesh for r in reqs], graph=graph)
+        self.live = [True] * len(reqs)
+        self.bytes = int(self.batch.stats().device_bytes)
+        for k, r in enumerate(reqs):
+            r.updates_per_step = self.batch.inst_updates(k)[0]
+        self.batch.velocity_change()  # snapshot at step 0
+
+    def any_live(self):
+        return any(self.live)
+
+
+class Server:
+    def __init__(self, storage="f16", check_every=100, tol=1e-6, max_steps=20000, mem_fraction=0.9,
+                 max_cohort=64, graph=True, keep_results=False):
+        self.storage, self.K, self.tol, self.max_steps = storage, check_every, tol, max_steps
+        self.max_cohort, self.graph, self.keep_results = max_cohort, graph, keep_results
+        free, total = C_mem_info()
+        self.budget = int(min(free, HBM_CAP) * mem_fraction)
+        self.used = 0
+        self.cohorts: list[Cohort] = []
+        self.done: list[Request] = []
+
+    def _admit(self, waiting, now):
+        batch = []
+        while waiting and waiting[0].arrival <= now and len(batch) < self.max_cohort:
+            r = waiting[0]
+            if r.mesh is None:
+                prepare(r, self.storage)
+            if r.footprint > self.budget:
+                raise RuntimeError(f"request {r.rid} needs {r.footprint} B, more than the whole budget")
+            if self.used + sum(q.footprint for q in batch) + r.footprint > self.budget:
+                break
+            batch.append(waiting.pop(0))
+        if batch:
+            c = Cohort(batch, self.graph)
+            t = time.perf_counter()
+            for r in batch:
+                r.admitted = t
+            self.used += c.bytes
+            self.cohorts.append(c)
+        return len(batch)
+
+    def _retire(self, c: Cohort, k, now, converged):
+        r = c.reqs[k]
+        r.finished = now
+        r.converged = converged
+        if self.keep_results:
+            r.result["f"] = c.batch.get_f(k)
+        c.batch.set_live(k, False)
+        c.live[k] = False
+        self.done.append(r)
+
+    def run(self, requests, wall_limit=None, t0=None):
+        """Serve `requests` (arrival times in seconds from start).  Returns
+        the list of finished requests; arrival/admit/finish are wall-clock
+        seconds from the start."""
+        waiting = sorted(requests, key=lambda r: r.arrival)
+        t0 = time.perf_counter() if t0 is None else t0
+        for r in waiting:
+            r.arrival += t0
+        ticks = 0
+        while waiting or self.cohorts:
+            now = time.perf_counter()
+            if wall_limit is not None and now - t0 > wall_limit:
+                break
+            self._admit(waiting, now)
+            if not self.cohorts:
+                if waiting:
+                    time.sleep(max(0.0, min(waiting[0].arrival - time.perf_counter(), 0.05)))
+                continue
+            for c in self.cohorts:
+                c.batch.step(self.K)
+            for c in self.cohorts:
+                du = c.batch.velocity_change()
+                now = time.perf_counter()
+                for k, r in enumerate(c.reqs):
+                    if not c.live[k]:
+                        continue
+                    r.steps += self.K
+                    if du[k] / self.K < self.tol:
+                        self._retire(c, k, now, True)
+                    elif r.steps >= self.max_steps:
+                        self._retire(c, k, now, False)
+            keep = []
+            for c in self.cohorts:
+                if c.any_live():
+                    keep.append(c)
+                else:
+                    self.used -= c.bytes
+                    c.batch.close()
+            self.cohorts = keep
+            ticks += 1
+            if ticks % 20 == 0:
+                log(f"t={time.perf_counter() - t0:.1f}s done={len(self.done)} waiting={len(waiting)} "
+                    f"cohorts={len(self.cohorts)} used={self.used / 2**30:.2f} GiB")
+        for r in self.done:
+            r.arrival -= t0
+            r.admitted -= t0
+            r.finished -= t0
+        self.unserved = len(waiting) + sum(sum(c.live) for c in self.cohorts)
+        return self.done
+
+
+def C_mem_info():
+    import ctypes as C
+    free, total = C.c_size_t(0), C.c_size_t(0)
+    check(lib().fl_mem_info(C.byref(free), C.byref(total)))
+    return int(free.value), int(total.value)
+
+
+def summary(done, wall, unserved):
+    lat = np.array([r.finished - r.arrival for r in done]) if done else np.zeros(1)
+    upd = sum(r.updates_per_step * r.steps for r in done)
+    return dict(served=len(done), unserved=unserved, converged=sum(r.converged for r in done),
+                p50_s=round(float(np.percentile(lat, 50)), 3), p99_s=round(float(np.percentile(lat, 99)), 3),
+                mean_queue_s=round(float(np.mean([r.admitted - r.arrival for r in done])) if done else 0.0, 3),
+                mlups=round(upd / wall / 1e6, 1), wall_s=round(wall, 2))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--rate", type=float, default=20.0)
+    ap.add_argument("--duration", type=float, default=600.0)
+    ap.add_argument("--scale", type=int, default=1)
+    ap.add_argument("--max-steps", type=int, default=20000)
+    ap.add_argument("--check-every", type=int, default=100)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--storage", default="f16")
+    ap.add_argument("--wall-limit", type=float, default=None)
+    ap.add_argument("--seed", type=int, default=42)
+    a = ap.parse_args()
+    check(lib().fl_init(0))
+    reqs = request_specs(a.rate, a.duration, a.seed, a.scale)
+    log(f"{len(reqs)} requests over {a.duration:g} s")
+    srv = Server(a.storage, a.check_every, a.tol, a.max_steps)
+    t0 = time.perf_counter()
+    done = srv.run(reqs, a.wall_limit, t0)
+    wall = time.perf_counter() - t0
+    out = dict(workload=f"Poisson {a.rate:g} req/s for {a.duration:g} s, 70% cavities / 30% spheres, "
+                        f"sizes / {a.scale}, steady state |du|/step < {a.tol:g} or {a.max_steps} steps, "
+                        f"{a.storage} storage", requests=len(reqs), budget_gib=round(srv.budget / 2**30, 1),
+               **summary(done, wall, srv.unserved))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -51,6 +51,10 @@ fl_status fl_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes, fl_stream 
 fl_status fl_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes, fl_stream s);
 fl_status fl_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes, fl_stream s);
 fl_status fl_memset(void* d_dst, int value, size_t bytes, fl_stream s);
+/* Plain vectorised device-to-device copy kernel (16 B per lane): the
+ * bandwidth reference for the HBM-bound kernels and the calibration of the
+ * profiler's HBM byte counters.  No counterpart in the reference. */
+fl_status fl_copy_kernel(void* d_dst, const void* d_src, size_t bytes, fl_stream s);
 fl_status fl_stream_create(fl_stream* s);
 fl_status fl_stream_destroy(fl_stream s);
 fl_status fl_stream_sync(fl_stream s);
