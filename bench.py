@@ -193,6 +193,17 @@ def main():
     alg_bytes = leaves[levels] * bpu
     achieved = alg_bytes / (ms_k * 1e-3) / 1e9
     log(f"level-{levels} collide: {ms_k:.3f} ms/launch, {achieved:.0f} GB/s algorithmic")
+    variant_ms = {}
+    for v, name in enumerate(variant_names()):
+        variant_ms[name] = round(solver.time_collide(levels, 10, v), 4)
+    log("finest-level collide variants (ms/launch): " + ", ".join(f"{k} {t:.3f}" for k, t in variant_ms.items()))
+    finest_variant = variant_names()[int(solver.stats().variant[levels])]
+    traffic = None
+    tf = ROOT / "profiles" / f"k_collide_sphere_{args.storage}_traffic.json"
+    if tf.exists() and sc == 1:
+        t = json.loads(tf.read_text())
+        if t.get("variant") == finest_variant:
+            traffic = int(t["traffic_bytes"])
 
     # bandwidth reference: a plain 16 B/lane copy kernel over 2 x 2 GiB
     copy_gbs = copy_bandwidth(_lib, L, stream)
@@ -229,9 +240,10 @@ def main():
         "drag_coefficient": round(float(cd), 4),
         "mass": mass,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"k_collide level {levels} ({leaves[levels]} leaves x {bpu} B)"},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": f"k_collide {finest_variant} level {levels} ({leaves[levels]} leaves x {bpu} B)"},
         "copy_kernel_gbs": round(copy_gbs, 1),
+        "finest_variant_ms": variant_ms,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

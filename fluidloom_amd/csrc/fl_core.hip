@@ -22,10 +22,18 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line) {
 }  // namespace fl
 
 namespace {
-// Plain streaming copy, 16 B per lane per iteration: the bandwidth
-// reference (and HBM-counter calibration) for the other kernels.
+// Plain streaming copy, 4 x 16 B per lane with all loads issued before the
+// stores: the bandwidth reference (and HBM-counter calibration) for the
+// other kernels.
 __global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+    const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
+    uint4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (base + 256 * k < n) v[k] = src[base + 256 * k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (base + 256 * k < n) dst[base + 256 * k] = v[k];
 }
 }  // namespace
 
@@ -104,8 +112,8 @@ fl_status fl_copy_kernel(void* d_dst, const void* d_src, size_t bytes, fl_stream
     FL_REQUIRE(((uintptr_t)d_dst | (uintptr_t)d_src) % 16 == 0, "fl_copy_kernel: pointers must be 16-byte aligned");
     const size_t n = bytes / 16;
     if (n == 0) return FL_OK;
-    size_t g = (n + 255) / 256;
-    if (g > 8192) g = 8192;
+    const size_t g = (n + 1023) / 1024;
+    FL_REQUIRE(g < (1ull << 31), "fl_copy_kernel: copy too large for one launch");
     hipLaunchKernelGGL(k_copy16, dim3((unsigned)g), dim3(256), 0, FL_STREAM(s), static_cast<const uint4*>(d_src),
                        static_cast<uint4*>(d_dst), n);
     FL_HIP(hipGetLastError());

@@ -40,17 +40,23 @@ __host__ __device__ constexpr double wq(int i) {
 // Storage codecs.
 template <typename T>
 struct Store;
+// store_nt: streaming store of the collide output (not re-read by the same
+// launch), kept out of the caches that serve neighbour re-reads.
 template <>
 struct Store<float> {
     using R = float;
     __device__ static inline float load(const float* p, size_t i, int) { return p[i]; }
     __device__ static inline void store(float* p, size_t i, int, float v) { p[i] = v; }
+    __device__ static inline void store_nt(float* p, size_t i, int, float v) { __builtin_nontemporal_store(v, p + i); }
 };
 template <>
 struct Store<double> {
     using R = double;
     __device__ static inline double load(const double* p, size_t i, int) { return p[i]; }
     __device__ static inline void store(double* p, size_t i, int, double v) { p[i] = v; }
+    __device__ static inline void store_nt(double* p, size_t i, int, double v) {
+        __builtin_nontemporal_store(v, p + i);
+    }
 };
 template <>
 struct Store<_Float16> {
@@ -60,6 +66,9 @@ struct Store<_Float16> {
     }
     __device__ static inline void store(_Float16* p, size_t i, int q, float v) {
         p[i] = (_Float16)(v - (float)wq(q));
+    }
+    __device__ static inline void store_nt(_Float16* p, size_t i, int q, float v) {
+        __builtin_nontemporal_store((_Float16)(v - (float)wq(q)), p + i);
     }
 };
 
