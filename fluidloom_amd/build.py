@@ -73,6 +73,8 @@ def build_cpp_tests(verbose: bool = True) -> Path:
 def build(jobs: int | None = None, verbose: bool = True) -> Path:
     OBJ.mkdir(exist_ok=True)
     LIBDIR.mkdir(exist_ok=True)
+    from .lattice import generate
+    generate()  # lattice constants header from the lattice declarations
     srcs = sources()
     jobs = jobs or min(8, len(srcs), os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(jobs) as ex:

@@ -1,11 +1,12 @@
 // D3Q19 lattice constants and per-cell device arithmetic.
 //
-// Velocity order: rest, +x, -x, +y, -y, +z, -z (the order of the lattice
-// fixture in tests/unit/parsing/test_parsing.cpp:23-29), then the twelve
-// edge directions in opposite pairs, so opp(i) = i + 1 for odd i and i - 1
-// for even i > 0 -- the table LatticeDescriptor::computeOpposites
-// (src/parsing/registry/LatticeDescriptor.cpp:30-49) derives.  Weights 1/3,
-// 1/18, 1/36; cs^2 = 1/3.
+// The velocities, weights and opposites come from lattice_gen.hpp, which
+// fluidloom_amd/lattice.py generates from its lattice declaration (the
+// HIP counterpart of LatticeDescriptor::generateOpenCLCode,
+// src/parsing/registry/LatticeDescriptor.cpp:51-80).  Velocity order: rest,
+// +x, -x, +y, -y, +z, -z (the lattice fixture in
+// tests/unit/parsing/test_parsing.cpp:23-29), then the twelve edge
+// directions in opposite pairs.  Weights 1/3, 1/18, 1/36; cs^2 = 1/3.
 //
 // Storage: populations are SoA f[q][slot] with slot = brick * 64 + local,
 // local = lx + 4 ly + 16 lz inside a 4x4x4 brick.  Three storage types:
@@ -15,27 +16,27 @@
 
 #include <cstdint>
 
+#include "lattice_gen.hpp"
+
 namespace fl {
 namespace lbm {
 
-constexpr int Q = 19;
+constexpr int Q = lattice::D3Q19_Q;
+static_assert(Q == 19, "the brick kernels are written for D3Q19");
 
-__host__ __device__ constexpr int cx(int i) {
-    return i == 1 || i == 7 || i == 9 || i == 11 || i == 13 ? 1
-         : (i == 2 || i == 8 || i == 10 || i == 12 || i == 14 ? -1 : 0);
+__host__ __device__ constexpr int cx(int i) { return lattice::D3Q19_C[i][0]; }
+__host__ __device__ constexpr int cy(int i) { return lattice::D3Q19_C[i][1]; }
+__host__ __device__ constexpr int cz(int i) { return lattice::D3Q19_C[i][2]; }
+__host__ __device__ constexpr int opp(int i) { return lattice::D3Q19_OPP[i]; }
+__host__ __device__ constexpr double wq(int i) { return lattice::D3Q19_W[i]; }
+
+// The kernels rely on opposite pairs (2k-1, 2k) after the rest velocity.
+constexpr bool pairs_adjacent() {
+    for (int i = 1; i < Q; ++i)
+        if (opp(i) != ((i & 1) ? i + 1 : i - 1)) return false;
+    return opp(0) == 0;
 }
-__host__ __device__ constexpr int cy(int i) {
-    return i == 3 || i == 7 || i == 10 || i == 15 || i == 17 ? 1
-         : (i == 4 || i == 8 || i == 9 || i == 16 || i == 18 ? -1 : 0);
-}
-__host__ __device__ constexpr int cz(int i) {
-    return i == 5 || i == 11 || i == 14 || i == 15 || i == 18 ? 1
-         : (i == 6 || i == 12 || i == 13 || i == 16 || i == 17 ? -1 : 0);
-}
-__host__ __device__ constexpr int opp(int i) { return i == 0 ? 0 : ((i & 1) ? i + 1 : i - 1); }
-__host__ __device__ constexpr double wq(int i) {
-    return i == 0 ? 1.0 / 3.0 : (i <= 6 ? 1.0 / 18.0 : 1.0 / 36.0);
-}
+static_assert(pairs_adjacent(), "lattice declaration must list opposite velocities in adjacent pairs");
 
 // Storage codecs.
 template <typename T>

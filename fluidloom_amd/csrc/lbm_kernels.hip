@@ -96,163 +96,6 @@ __global__ __launch_bounds__(64 * BPW) void k_collide(LevelDev L, const T* __res
     }
 }
 
-// Four consecutive slots (one x-row of a brick) of one population plane.
-typedef float v4f __attribute__((ext_vector_type(4)));
-typedef unsigned int v2u __attribute__((ext_vector_type(2)));
-template <typename T>
-struct Row;
-template <>
-struct Row<float> {
-    __device__ static inline void load(const float* p, size_t idx, int, float* v) {
-        const v4f t = *reinterpret_cast<const v4f*>(p + idx);
-        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
-    }
-    __device__ static inline void store(float* p, size_t idx, int, const float* v) {
-        const v4f t = {v[0], v[1], v[2], v[3]};
-        __builtin_nontemporal_store(t, reinterpret_cast<v4f*>(p + idx));
-    }
-};
-template <>
-struct Row<_Float16> {
-    __device__ static inline void load(const _Float16* p, size_t idx, int q, float* v) {
-        const v2u t = *reinterpret_cast<const v2u*>(p + idx);
-        const _Float16* h = reinterpret_cast<const _Float16*>(&t);
-        const float w = (float)wq(q);
-        v[0] = w + (float)h[0]; v[1] = w + (float)h[1]; v[2] = w + (float)h[2]; v[3] = w + (float)h[3];
-    }
-    __device__ static inline void store(_Float16* p, size_t idx, int q, const float* v) {
-        const float w = (float)wq(q);
-        v2u t;
-        _Float16* h = reinterpret_cast<_Float16*>(&t);
-        h[0] = (_Float16)(v[0] - w); h[1] = (_Float16)(v[1] - w); h[2] = (_Float16)(v[2] - w); h[3] = (_Float16)(v[3] - w);
-        __builtin_nontemporal_store(t, reinterpret_cast<v2u*>(p + idx));
-    }
-};
-
-// Row form of the collide-stream: a wave covers 4 bricks, lane = one x-row
-// of 4 cells (16 lanes per brick).  Every population plane is pulled as one
-// 16 B (FP32) / 8 B (FP16) row load, plus one element of the x-neighbour
-// brick for the 10 directions with an x component; the brick neighbour rows
-// of the wave's 4 bricks are staged in LDS.  Same per-cell arithmetic as
-// collide_brick, so the results are identical.
-template <typename T, int WPG>
-__device__ inline void collide_rows_group(const LevelDev& L, const T* __restrict__ A, T* __restrict__ B, unsigned g,
-                                          int* __restrict__ sn) {
-    using R = typename Store<T>::R;
-    const int lane = (int)(threadIdx.x & 63);
-    const int wave = (int)(threadIdx.x >> 6);
-    const int b0 = (int)(g * (4 * WPG)) + wave * 4;
-    for (int e = lane; e < 4 * 27; e += 64) {
-        const int bj = b0 + e / 27;
-        sn[e] = bj < L.nactive ? L.nbr[(size_t)bj * 27 + e % 27] : -1;
-    }
-    __syncthreads();
-    const int j = lane >> 4, r = lane & 15;
-    const int b = b0 + j;
-    bool valid = b < L.nactive;
-    int inst = 0;
-    if (valid) {
-        inst = L.binst[b];
-        valid = L.inst[inst].live != 0;
-    }
-    if (valid) {
-        const int ly = r & 3, lz = r >> 2;
-        const size_t S = L.S;
-        const size_t rowslot = (size_t)b * 64 + 4 * r;
-        const bool simple = (L.bflags[b] & 1u) != 0;
-        const int* nrow = sn + j * 27;
-        R f[4][Q];
-#pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            const int py = ly - cy(i), pz = lz - cz(i);
-            const int nbi = 13 + 3 * (py >> 2) + 9 * (pz >> 2);
-            const int rr = (py & 3) + 4 * (pz & 3);
-            const int nb = nrow[nbi];
-            R v[4] = {0, 0, 0, 0};
-            if (nb >= 0) {
-                if constexpr (sizeof(T) == 8) {
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) v[c] = Store<T>::load(A, (size_t)i * S + (size_t)nb * 64 + 4 * rr + c, i);
-                } else {
-                    Row<T>::load(A, (size_t)i * S + (size_t)nb * 64 + 4 * rr, i, v);
-                }
-            }
-            if (cx(i) == 0) {
-#pragma unroll
-                for (int c = 0; c < 4; ++c) f[c][i] = v[c];
-            } else if (cx(i) > 0) {
-                const int nx = nrow[nbi - 1];
-                f[0][i] = nx >= 0 ? Store<T>::load(A, (size_t)i * S + (size_t)nx * 64 + 4 * rr + 3, i) : (R)0;
-                f[1][i] = v[0];
-                f[2][i] = v[1];
-                f[3][i] = v[2];
-            } else {
-                const int nx = nrow[nbi + 1];
-                f[0][i] = v[1];
-                f[1][i] = v[2];
-                f[2][i] = v[3];
-                f[3][i] = nx >= 0 ? Store<T>::load(A, (size_t)i * S + (size_t)nx * 64 + 4 * rr, i) : (R)0;
-            }
-        }
-        bool act[4] = {true, true, true, true};
-        if (!simple) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const uint64_t w = L.cflags[rowslot + c];
-                act[c] = (w & kActiveBit) != 0;
-                if (!act[c]) continue;
-#pragma unroll
-                for (int i = 1; i < Q; ++i) {
-                    const unsigned code = (unsigned)(w >> (2 * i)) & 3u;
-                    if (code != kLinkRegular)
-                        f[c][i] = special_link<T, R>(L, A, b, rowslot + c, i, code, c - cx(i), ly - cy(i), lz - cz(i));
-                }
-            }
-        }
-        const R itau = inv_tau_of<R>(L, inst);
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-            if (act[c]) bgk(f[c], itau);
-        bool full = act[0] && act[1] && act[2] && act[3];
-        if constexpr (sizeof(T) != 8) {
-            if (full) {
-#pragma unroll
-                for (int i = 0; i < Q; ++i) {
-                    const R v[4] = {f[0][i], f[1][i], f[2][i], f[3][i]};
-                    Row<T>::store(B, (size_t)i * S + rowslot, i, v);
-                }
-            }
-        } else {
-            full = false;
-        }
-        if (!full) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-                if (act[c])
-#pragma unroll
-                    for (int i = 0; i < Q; ++i) Store<T>::store_nt(B, (size_t)i * S + rowslot + c, i, f[c][i]);
-        }
-    }
-    __syncthreads();  // sn is rewritten by the next group
-}
-
-template <typename T, int WPG, int MODE>
-__global__ __launch_bounds__(64 * WPG) void k_collide_rows(LevelDev L, const T* __restrict__ A, T* __restrict__ B,
-                                                           unsigned ngroups) {
-    __shared__ int snbr[WPG][4 * 27];
-    int* sn = snbr[threadIdx.x >> 6];
-    if (MODE == 2) {
-        const unsigned wg = xcd_remap(blockIdx.x, gridDim.x);
-        const unsigned per = (ngroups + gridDim.x - 1) / gridDim.x;
-        const unsigned g0 = wg * per;
-        const unsigned g1 = g0 + per < ngroups ? g0 + per : ngroups;
-        for (unsigned g = g0; g < g1; ++g) collide_rows_group<T, WPG>(L, A, B, g, sn);
-    } else {
-        const unsigned g = MODE == 1 ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
-        collide_rows_group<T, WPG>(L, A, B, g, sn);
-    }
-}
-
 template <typename T>
 __global__ __launch_bounds__(kWG) void k_capture(LevelDev L, const T* __restrict__ A,
                                                  const uint32_t* __restrict__ cap_slots, uint32_t ncap,
@@ -712,27 +555,6 @@ int device_cus() {
     return cus;
 }
 
-template <typename T, int WPG, int MODE>
-hipError_t collide_rows_variant(const LevelDev& L, const T* A, T* B, hipStream_t s) {
-    const unsigned ngroups = (unsigned)((L.nactive + 4 * WPG - 1) / (4 * WPG));
-    unsigned grid = ngroups;
-    if (MODE == 2) {
-        static int per_cu = 0;
-        if (per_cu == 0) {
-            int nb = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide_rows<T, WPG, MODE>, 64 * WPG, 0) !=
-                    hipSuccess ||
-                nb <= 0)
-                nb = 4 / WPG > 0 ? 4 / WPG : 1;
-            per_cu = nb;
-        }
-        const unsigned cap = (unsigned)(per_cu * device_cus());
-        if (grid > cap) grid = cap;
-    }
-    hipLaunchKernelGGL((k_collide_rows<T, WPG, MODE>), dim3(grid), dim3(64 * WPG), 0, s, L, A, B, ngroups);
-    return hipGetLastError();
-}
-
 template <typename T, int BPW, int MODE>
 hipError_t collide_variant(const LevelDev& L, const T* A, T* B, hipStream_t s) {
     const unsigned ngroups = (unsigned)((L.nactive + BPW - 1) / BPW);
@@ -756,15 +578,12 @@ hipError_t collide_variant(const LevelDev& L, const T* A, T* B, hipStream_t s) {
 
 }  // namespace
 
-int collide_variants() { return 18; }
+int collide_variants() { return 9; }
 
 const char* collide_variant_name(int v) {
-    static const char* names[] = {"bpw1",          "bpw2",          "bpw4",          "bpw1-xcd",
-                                  "bpw2-xcd",      "bpw4-xcd",      "bpw1-persist",  "bpw2-persist",
-                                  "bpw4-persist",  "rows-w1",       "rows-w2",       "rows-w4",
-                                  "rows-w1-xcd",   "rows-w2-xcd",   "rows-w4-xcd",   "rows-w1-persist",
-                                  "rows-w2-persist", "rows-w4-persist"};
-    return v >= 0 && v < 18 ? names[v] : "?";
+    static const char* names[] = {"bpw1",     "bpw2",     "bpw4",         "bpw1-xcd",     "bpw2-xcd",
+                                  "bpw4-xcd", "bpw1-persist", "bpw2-persist", "bpw4-persist"};
+    return v >= 0 && v < 9 ? names[v] : "?";
 }
 
 template <typename T>
@@ -779,16 +598,7 @@ hipError_t launch_collide(const LevelDev& L, const T* A, T* B, int variant, hipS
         case 5: return collide_variant<T, 4, 1>(L, A, B, s);
         case 6: return collide_variant<T, 1, 2>(L, A, B, s);
         case 7: return collide_variant<T, 2, 2>(L, A, B, s);
-        case 8: return collide_variant<T, 4, 2>(L, A, B, s);
-        case 9: return collide_rows_variant<T, 1, 0>(L, A, B, s);
-        case 10: return collide_rows_variant<T, 2, 0>(L, A, B, s);
-        case 11: return collide_rows_variant<T, 4, 0>(L, A, B, s);
-        case 12: return collide_rows_variant<T, 1, 1>(L, A, B, s);
-        case 13: return collide_rows_variant<T, 2, 1>(L, A, B, s);
-        case 14: return collide_rows_variant<T, 4, 1>(L, A, B, s);
-        case 15: return collide_rows_variant<T, 1, 2>(L, A, B, s);
-        case 16: return collide_rows_variant<T, 2, 2>(L, A, B, s);
-        default: return collide_rows_variant<T, 4, 2>(L, A, B, s);
+        default: return collide_variant<T, 4, 2>(L, A, B, s);
     }
 }
 
