@@ -3,9 +3,9 @@
 //
 //   k_collide   fused pull-stream + BGK over the active bricks: one 64-lane
 //               wave per 4x4x4 brick, lane = cell.  The brick's 27-entry
-//               neighbour row sits in lanes 0..26 and is fetched with
-//               __shfl, so a regular pull costs one HBM-streamed population
-//               load; cells of "simple" bricks (all 64 active, no wall links)
+//               neighbour row is held in scalar registers, so a regular
+//               pull costs one population load plus a few lane selects;
+//               cells of "simple" bricks (all 64 active, no wall links)
 //               never read their link word.  Workgroups are remapped so that
 //               each XCD processes one contiguous Hilbert-ordered range of
 //               bricks, keeping cross-brick re-reads inside one L2.
@@ -41,16 +41,27 @@ __device__ inline R inv_tau_of(const LevelDev& L, int inst) {
     return (R)L.itau[inst];
 }
 
+// One wave per brick, lane = cell.  The brick's 27 neighbour entries are
+// wave-uniform loads (scalar registers); for direction i a lane's source
+// brick is one of at most four of them, chosen by whether the pulled
+// position leaves the brick along each axis.  Offsets inside a population
+// plane are 32-bit (the layout keeps 19 * slots below 2^32), on a
+// wave-uniform plane base.
 template <typename T>
-__device__ inline void collide_brick(const LevelDev& L, const T* __restrict__ A, T* __restrict__ B, int b) {
+__device__ __attribute__((always_inline)) inline void collide_brick(const LevelDev& L, const T* __restrict__ A,
+                                                                    T* __restrict__ B, int b_in) {
     using R = typename Store<T>::R;
-    if (b >= L.nactive) return;  // wave-uniform
+    const int b = __builtin_amdgcn_readfirstlane(b_in);
+    if (b >= L.nactive) return;
     const int inst = L.binst[b];
     if (!L.inst[inst].live) return;
     const int lane = (int)(threadIdx.x & 63);
-    const int nbrv = lane < 27 ? L.nbr[b * 27 + lane] : -1;
+    const int* __restrict__ row = L.nbr + (size_t)b * 27;
+    int NB[27];
+#pragma unroll
+    for (int k = 0; k < 27; ++k) NB[k] = row[k];
     const bool simple = (L.bflags[b] & 1u) != 0;
-    const size_t slot = (size_t)b * 64 + lane;
+    const unsigned slot = (unsigned)b * 64u + (unsigned)lane;
     const uint64_t w = simple ? 0ull : L.cflags[slot];
     const bool active = simple || (w & kActiveBit);
     const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
@@ -59,14 +70,18 @@ __device__ inline void collide_brick(const LevelDev& L, const T* __restrict__ A,
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         const int px = lx - cx(i), py = ly - cy(i), pz = lz - cz(i);
-        const int nbi = 13 + (px >> 2) + 3 * (py >> 2) + 9 * (pz >> 2);
-        const int nb = __shfl(nbrv, nbi, 64);
-        const size_t src = (size_t)nb * 64 + (size_t)((px & 3) + 4 * (py & 3) + 16 * (pz & 3));
+        const bool ex = (unsigned)px > 3u, ey = (unsigned)py > 3u, ez = (unsigned)pz > 3u;
+        const int kx = -cx(i), ky = -3 * cy(i), kz = -9 * cz(i);
+        const int n000 = NB[13], n100 = NB[13 + kx], n010 = NB[13 + ky], n110 = NB[13 + kx + ky];
+        const int n001 = NB[13 + kz], n101 = NB[13 + kx + kz], n011 = NB[13 + ky + kz], n111 = NB[13 + kx + ky + kz];
+        const int nb = ez ? (ey ? (ex ? n111 : n011) : (ex ? n101 : n001))
+                          : (ey ? (ex ? n110 : n010) : (ex ? n100 : n000));
+        const unsigned src = (unsigned)nb * 64u + (unsigned)((px & 3) + 4 * (py & 3) + 16 * (pz & 3));
         f[i] = 0;
         if (active) {
             const unsigned code = (unsigned)(w >> (2 * i)) & 3u;
             if (code == kLinkRegular)
-                f[i] = Store<T>::load(A, (size_t)i * S + src, i);
+                f[i] = Store<T>::load(A + (size_t)i * S, src, i);
             else
                 f[i] = special_link<T, R>(L, A, b, slot, i, code, px, py, pz);
         }
@@ -74,7 +89,7 @@ __device__ inline void collide_brick(const LevelDev& L, const T* __restrict__ A,
     if (!active) return;
     bgk(f, inv_tau_of<R>(L, inst));
 #pragma unroll
-    for (int i = 0; i < Q; ++i) Store<T>::store_nt(B, (size_t)i * S + slot, i, f[i]);
+    for (int i = 0; i < Q; ++i) Store<T>::store_nt(B + (size_t)i * S, slot, i, f[i]);
 }
 
 // MODE 0: workgroup g takes brick group g; 1: the same with the XCD-aware

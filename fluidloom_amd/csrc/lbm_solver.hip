@@ -1122,24 +1122,33 @@ fl_status remap(fl_lbm_solver* O, fl_lbm_solver* N) {
     return FL_OK;
 }
 
+// Back-to-back launches of one level's collide-stream, timed with HIP
+// events on the solver's stream.  Every launch reads the current buffer and
+// writes the other one; the other buffer (which the next step's captures and
+// fills read) is saved first and restored after, so the solver state is
+// unchanged.
 template <typename T>
 fl_status time_collide(fl_lbm_solver* S, int level, int variant, int iters, float* ms) {
     DevLevel& D = S->d[level];
-    hipEvent_t a, b;
-    FL_HIP(hipEventCreate(&a));
-    FL_HIP(hipEventCreate(&b));
-    hipError_t e = hipEventRecord(a, S->stream);
-    for (int it = 0; it < iters && e == hipSuccess; ++it) {
+    const size_t bytes = D.ld.S * Q * sizeof(T);
+    void* save = nullptr;
+    FL_HIP(hipMalloc(&save, bytes));
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipMemcpyAsync(save, D.buf[1 - D.cur], bytes, hipMemcpyDeviceToDevice, S->stream);
+    if (e == hipSuccess) e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = hipEventRecord(a, S->stream);
+    for (int it = 0; it < iters && e == hipSuccess; ++it)
         e = launch_collide<T>(D.ld, static_cast<const T*>(D.buf[D.cur]), static_cast<T*>(D.buf[1 - D.cur]), variant,
                               S->stream);
-        D.cur ^= 1;
-    }
     if (e == hipSuccess) e = hipEventRecord(b, S->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(b);
+    if (e == hipSuccess) e = hipMemcpyAsync(D.buf[1 - D.cur], save, bytes, hipMemcpyDeviceToDevice, S->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(S->stream);
     float t = 0;
     if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    (void)hipFree(save);
     FL_HIP(e);
     *ms = t / iters;
     return FL_OK;
@@ -1438,15 +1447,13 @@ fl_status fl_lbm_autotune(fl_lbm_solver* S, int iters, int* chosen) {
             if (chosen) chosen[l] = S->variant[l];
             continue;
         }
-        // an even number of launches per variant leaves the level's state
-        // parity unchanged; one warm-up pair per variant
-        const int it2 = iters + (iters & 1);
+        // timing leaves the solver state unchanged; one warm-up launch per variant
         int best = S->variant[l];
         float best_ms = 1e30f;
         for (int v = 0; v < collide_variants(); ++v) {
             float ms;
-            fl_status r = time_any(S, l, v, 2, &ms);
-            if (!r) r = time_any(S, l, v, it2, &ms);
+            fl_status r = time_any(S, l, v, 1, &ms);
+            if (!r) r = time_any(S, l, v, iters, &ms);
             if (r) return r;
             if (ms < best_ms) {
                 best_ms = ms;

@@ -81,10 +81,10 @@ def request_specs(rate, duration, seed=42, scale=1, levels_max=3):
             out.append(Request(rid, t, "cavity", spec))
         else:
             f = float(rng.uniform(1.0, 2.0))
-            nx = int(max(16, 4 * round(128 * f / scale / 4)))
+            ny = int(max(8, 4 * round(64 * f / scale / 4)))
             r = float(rng.uniform(8.0, 16.0)) / scale
             re = float(rng.uniform(50.0, 200.0))
-            spec = dict(nx=nx, ny=nx // 2, nz=nx // 2, r=r, re=re, u_in=0.05, max_level=nlev - 1)
+            spec = dict(nx=2 * ny, ny=ny, nz=ny, r=r, re=re, u_in=0.05, max_level=nlev - 1)
             out.append(Request(rid, t, "sphere", spec))
         rid += 1
     return out

@@ -362,8 +362,9 @@ fl_status fl_lbm_autotune(fl_lbm_solver* solver, int iters, int* h_chosen);
 fl_status fl_lbm_time_collide_variant(fl_lbm_solver* solver, int level, int variant, int iters,
                                       float* ms_per_launch);
 /* Measurement: `iters` back-to-back collide-stream launches of one level on
- * the solver's stream, timed with HIP events (advances that level's state;
- * a measurement of the kernel alone, not part of a physical step). */
+ * the solver's stream, timed with HIP events.  The solver state is left
+ * unchanged (the output buffer is saved and restored around the launches,
+ * which needs one level buffer of temporary device memory). */
 fl_status fl_lbm_time_collide(fl_lbm_solver* solver, int level, int iters, float* ms_per_launch);
 
 #ifdef __cplusplus
