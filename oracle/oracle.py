@@ -16,11 +16,24 @@ How each oracle function is pinned (besides the GPU comparison):
   split / merge           reference unit-test fixtures (SplitEngineTest,
                           MergeEngineTest, AdaptationIntegrationTest) +
                           brute-force sibling search
-  balance                 reference fixture (BalanceEnforcerTest) + brute-force
-                          2:1 checker on the converged output
+  balance                 reference fixture (BalanceEnforcerTest); both rules
+                          against an exhaustive restatement of the iteration
+                          (every cell tested against every cell, flags and
+                          counters compared); brute-force 2:1 checker on the
+                          converged output
+  key63 / cell key        prefix alignment; the packing closed form
+  compact_append          closed form (numpy selection + concatenation) and
+                          the split child placement rule
+  merge_fields            closed form (sum in sibling order, x 0.125 rule 0)
+  gen_cells / gen_flags   define the synthetic micro-benchmark input (no
+                          reference counterpart); checked against the
+                          workload's definition (ranges, Bernoulli rates,
+                          merges only on complete sibling groups)
   lbm                     closed forms: equilibrium moments, conservation,
                           rest state, Couette profile, Taylor-Green decay,
-                          uniform flow through coarse/fine interfaces
+                          uniform flow through coarse/fine interfaces.  The
+                          reference has no lattice step, so its parity with
+                          the reference is unpinned by construction.
 """
 from __future__ import annotations
 
