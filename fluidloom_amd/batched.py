@@ -151,14 +151,20 @@ def run(instances=256, steps=2000, regrid_every=0, plain_sample=32, plain_steps=
         log(f"step {done}/{steps}: " + ", ".join(f"{st} {upd_step * done / max(t, 1e-9) / 1e6:.0f} MLUPS"
                                                  for st, t in times.items()))
     res = {}
+    errs = {}
     for st in ("f32", "f16"):
         e = velocity_error(batches[st], batches["f64"])
+        errs[st] = e
         m = batches[st].totals_batch()[0]
         drift = np.abs(m - m0[st]) / m0[st]
         res[st] = dict(mlups=round(upd_step * steps / times[st] / 1e6, 1), seconds=round(times[st], 3),
                        solves_per_s=round(instances / times[st], 3),
                        vel_l2_max=float(e.max()), vel_l2_median=float(np.median(e)),
                        mass_drift_max=float(drift.max()))
+    worst = np.argsort(-errs["f16"])[:5]
+    res["f16_worst"] = [dict(instance=int(k), **specs[k], vel_l2_f16=float(errs["f16"][k]),
+                             vel_l2_f32=float(errs["f32"][k])) for k in worst]
+    res["f16_frac_below_1e-3"] = float((errs["f16"] <= 1e-3).mean())
     m64 = batches["f64"].totals_batch()[0]
     res["f64"] = dict(mlups=round(upd_step * steps / times["f64"] / 1e6, 1), seconds=round(times["f64"], 3),
                       mass_drift_max=float((np.abs(m64 - m0["f64"]) / m0["f64"]).max()))
