@@ -273,7 +273,9 @@ typedef struct fl_lbm_solver fl_lbm_solver;
 fl_status fl_lbm_create(const fl_lbm_params* p, const int32_t* h_x, const int32_t* h_y,
                         const int32_t* h_z, const uint8_t* h_level, const uint8_t* h_state,
                         size_t n, fl_stream s, fl_lbm_solver** out);
-/* A batch of independent instances in one layout: instance k owns leaves
+/* (No counterpart in the reference: batches, graph replay, autotuning and the
+ * serving queries below are this library's own.)
+ * A batch of independent instances in one layout: instance k owns leaves
  * [offsets[k], offsets[k+1]) of the concatenated arrays and its own
  * params[k] (grid, tau, boundaries, levels); every instance uses the same
  * storage.  Bricks of all instances are concatenated per level (grouped by
