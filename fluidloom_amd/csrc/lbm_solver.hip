@@ -104,29 +104,74 @@ inline uint64_t bkey(uint32_t inst, int32_t bx, int32_t by, int32_t bz) {
 inline uint32_t bkey_inst(uint64_t k) { return (uint32_t)(k >> 48); }
 inline int32_t bkey_c(uint64_t k, int a) { return (int32_t)((k >> (32 - 16 * a)) & 0xFFFF); }
 
+// parallel_chunks(n, f): f(begin, end) over [0, n) in chunks, on up to 16
+// threads; chunk results are combined by the caller.
+template <class F>
+void parallel_chunks(size_t n, size_t chunk, F&& f) {
+    unsigned nt = std::thread::hardware_concurrency();
+    if (nt == 0) nt = 4;
+    if (nt > 16) nt = 16;
+    const size_t nchunks = (n + chunk - 1) / chunk;
+    if (nchunks <= 1 || nt == 1) {
+        for (size_t c = 0; c < nchunks; ++c) f(c * chunk, std::min(n, (c + 1) * chunk));
+        return;
+    }
+    std::vector<std::thread> th;
+    std::atomic<size_t> next{0};
+    for (unsigned t = 0; t < nt && t < nchunks; ++t)
+        th.emplace_back([&] {
+            for (;;) {
+                const size_t c = next.fetch_add(1);
+                if (c >= nchunks) break;
+                f(c * chunk, std::min(n, (c + 1) * chunk));
+            }
+        });
+    for (auto& t : th) t.join();
+}
+
+// Brick index of one instance at one level: a dense grid of brick numbers
+// (-1 = none) when the level's brick grid is small enough, else the level's
+// hash map.
+constexpr size_t kDenseBricksMax = size_t(1) << 27;
+
+struct BrickGrid {
+    int32_t n[3] = {0, 0, 0};
+    bool dense = false;
+    std::vector<int32_t> idx;
+    size_t at(int32_t bx, int32_t by, int32_t bz) const { return ((size_t)bz * n[1] + by) * n[0] + bx; }
+    size_t cells() const { return (size_t)n[0] * n[1] * n[2]; }
+};
+
 struct HostLevel {
     int level = 0;
     std::vector<std::array<int32_t, 3>> iext;  // [instance] extents in cells at this level
     uint32_t nb = 0, nactive = 0;
     uint64_t nfluid = 0;
     std::vector<uint64_t> nfluid_inst;
-    std::unordered_map<uint64_t, int32_t> bmap;
+    std::vector<BrickGrid> grid;                // [instance]
+    std::unordered_map<uint64_t, int32_t> bmap;  // bricks of instances without a dense grid
     std::vector<int32_t> bcoord, nbr;
     std::vector<uint16_t> binst;
     std::vector<uint32_t> bflags;
     std::vector<uint64_t> cflags;
     std::vector<uint8_t> stype, needed;
-    std::vector<int64_t> c2fsrc;
     std::vector<uint32_t> c2f_ghost, c2f_cap, cap_slots, f2c_ghost, f2c_child, force_links;
     std::vector<uint32_t> abeg;  // [ninst+1] active-brick range of each instance
     std::vector<uint32_t> lbeg;  // [ninst+1] solid-link range of each instance
 
+    int32_t brick_at(int inst, int32_t bx, int32_t by, int32_t bz) const {
+        const BrickGrid& g = grid[(size_t)inst];
+        if (bx < 0 || by < 0 || bz < 0 || bx >= g.n[0] || by >= g.n[1] || bz >= g.n[2]) return -1;
+        if (g.dense) return g.idx[g.at(bx, by, bz)];
+        auto it = bmap.find(bkey((uint32_t)inst, bx, by, bz));
+        return it == bmap.end() ? -1 : it->second;
+    }
     int64_t slot_at(int inst, int32_t x, int32_t y, int32_t z) const {
         const auto& e = iext[(size_t)inst];
         if (x < 0 || y < 0 || z < 0 || x >= e[0] || y >= e[1] || z >= e[2]) return -1;
-        auto it = bmap.find(bkey((uint32_t)inst, x >> 2, y >> 2, z >> 2));
-        if (it == bmap.end()) return -1;
-        return (int64_t)it->second * 64 + (x & 3) + 4 * (y & 3) + 16 * (z & 3);
+        const int32_t b = brick_at(inst, x >> 2, y >> 2, z >> 2);
+        if (b < 0) return -1;
+        return (int64_t)b * 64 + (x & 3) + 4 * (y & 3) + 16 * (z & 3);
     }
     uint8_t type_at(int inst, int32_t x, int32_t y, int32_t z) const {
         int64_t s = slot_at(inst, x, y, z);
@@ -299,103 +344,168 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
             H.iext[k] = {p.nx0 << l, p.ny0 << l, p.nz0 << l};
         }
     }
-    // 1. leaf bricks per level
-    std::vector<std::vector<uint64_t>> active_keys(NL), leaf_keys(NL);
-    {
-        std::vector<uint64_t> last(NL, ~0ull), lastf(NL, ~0ull);
-        for (size_t k = 0; k < n; ++k) {
-            const int l = LV[k];
-            const int in = linst[k];
-            if (l > S->inst[in].p.max_level) return mesh_error("leaf level above the instance's max_level");
-            const auto& e = S->h[l].iext[in];
-            if (X[k] < 0 || Y[k] < 0 || Z[k] < 0 || X[k] >= e[0] || Y[k] >= e[1] || Z[k] >= e[2])
-                return mesh_error("leaf outside the domain");
-            uint64_t bk = bkey((uint32_t)in, X[k] >> 2, Y[k] >> 2, Z[k] >> 2);
-            if (bk != last[l]) { leaf_keys[l].push_back(bk); last[l] = bk; }
-            if (ST[k] == 0 && bk != lastf[l]) { active_keys[l].push_back(bk); lastf[l] = bk; }
-        }
+    // 1-2. bricks per level: the leaf bricks plus the 26-neighbour dilation of
+    //      the active (fluid-holding) bricks.  Instances whose brick grid is
+    //      small enough mark a dense byte grid (bit 0 leaf brick, bit 1
+    //      active, bit 2 dilation); the others collect and sort keys.
+    for (size_t j = 0; j < n; ++j) {
+        const int in = linst[j];
+        if (LV[j] > S->inst[in].p.max_level) return mesh_error("leaf level above the instance's max_level");
+        const auto& e = S->h[LV[j]].iext[in];
+        if (X[j] < 0 || Y[j] < 0 || Z[j] < 0 || X[j] >= e[0] || Y[j] >= e[1] || Z[j] >= e[2])
+            return mesh_error("leaf outside the domain");
     }
-    // 2. bricks = leaf bricks + 26-neighbour dilation of active bricks
     for (int l = 0; l < NL; ++l) {
         HostLevel& H = S->h[l];
-        auto& ak = active_keys[l];
-        std::sort(ak.begin(), ak.end());
-        ak.erase(std::unique(ak.begin(), ak.end()), ak.end());
-        std::vector<uint64_t> all = leaf_keys[l];
-        all.reserve(all.size() + ak.size() * 27);
-        for (uint64_t k : ak) {
-            const uint32_t in = bkey_inst(k);
-            const fl_lbm_params& p = S->inst[in].p;
-            const std::array<int32_t, 3> nbx = {H.iext[in][0] / 4, H.iext[in][1] / 4, H.iext[in][2] / 4};
-            for (int dz = -1; dz <= 1; ++dz)
-                for (int dy = -1; dy <= 1; ++dy)
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        int32_t c[3] = {bkey_c(k, 0) + dx, bkey_c(k, 1) + dy, bkey_c(k, 2) + dz};
-                        if (wrap_brick(p, nbx, c)) all.push_back(bkey(in, c[0], c[1], c[2]));
-                    }
-        }
-        std::sort(all.begin(), all.end());
-        all.erase(std::unique(all.begin(), all.end()), all.end());
-        // order: active first, then by instance, then along the Hilbert curve
+        H.grid.assign(NI, BrickGrid());
         struct E { int act; uint32_t inst; uint64_t h; uint64_t k; };
-        std::vector<E> es(all.size());
-        parallel_for(all.size(), [&](size_t i) {
-            uint64_t k = all[i];
-            es[i] = E{std::binary_search(ak.begin(), ak.end(), k) ? 0 : 1, bkey_inst(k),
-                      host_encode((uint32_t)bkey_c(k, 0), (uint32_t)bkey_c(k, 1), (uint32_t)bkey_c(k, 2), 16), k};
-        });
+        std::vector<E> es;
+        for (int in = 0; in < NI; ++in) {
+            const fl_lbm_params& p = S->inst[in].p;
+            BrickGrid& g = H.grid[in];
+            for (int a = 0; a < 3; ++a) g.n[a] = H.iext[in][a] / 4;
+            const std::array<int32_t, 3> nbx = {g.n[0], g.n[1], g.n[2]};
+            const size_t j0 = S->inst[in].first, j1 = j0 + S->inst[in].count;
+            if (l > p.max_level) continue;
+            g.dense = g.cells() <= kDenseBricksMax;
+            if (g.dense) {
+                std::vector<uint8_t> fl(g.cells(), 0);
+                parallel_chunks(j1 - j0, 1 << 16, [&](size_t b0, size_t b1) {
+                    for (size_t j = j0 + b0; j < j0 + b1; ++j) {
+                        if (LV[j] != l) continue;
+                        const size_t c = g.at(X[j] >> 2, Y[j] >> 2, Z[j] >> 2);
+                        const uint8_t bits = ST[j] == 0 ? 3 : 1;
+                        if ((__atomic_load_n(&fl[c], __ATOMIC_RELAXED) & bits) != bits)
+                            __atomic_fetch_or(&fl[c], bits, __ATOMIC_RELAXED);
+                    }
+                });
+                parallel_chunks(g.cells(), 1 << 16, [&](size_t c0, size_t c1) {
+                    for (size_t c = c0; c < c1; ++c) {
+                        if (!(__atomic_load_n(&fl[c], __ATOMIC_RELAXED) & 2)) continue;
+                        const int32_t bx = (int32_t)(c % g.n[0]), by = (int32_t)((c / g.n[0]) % g.n[1]),
+                                      bz = (int32_t)(c / ((size_t)g.n[0] * g.n[1]));
+                        for (int dz = -1; dz <= 1; ++dz)
+                            for (int dy = -1; dy <= 1; ++dy)
+                                for (int dx = -1; dx <= 1; ++dx) {
+                                    int32_t q[3] = {bx + dx, by + dy, bz + dz};
+                                    if (!wrap_brick(p, nbx, q)) continue;
+                                    uint8_t& f = fl[g.at(q[0], q[1], q[2])];
+                                    if (!(__atomic_load_n(&f, __ATOMIC_RELAXED) & 4))
+                                        __atomic_fetch_or(&f, (uint8_t)4, __ATOMIC_RELAXED);
+                                }
+                    }
+                });
+                // collect in grid order, chunk by chunk
+                const size_t chunk = 1 << 18, nch = (g.cells() + chunk - 1) / chunk;
+                std::vector<std::vector<E>> part(nch);
+                parallel_chunks(g.cells(), chunk, [&](size_t c0, size_t c1) {
+                    auto& v = part[c0 / chunk];
+                    for (size_t c = c0; c < c1; ++c) {
+                        if (!fl[c]) continue;
+                        const int32_t bx = (int32_t)(c % g.n[0]), by = (int32_t)((c / g.n[0]) % g.n[1]),
+                                      bz = (int32_t)(c / ((size_t)g.n[0] * g.n[1]));
+                        v.push_back(E{(fl[c] & 2) ? 0 : 1, (uint32_t)in,
+                                      host_encode((uint32_t)bx, (uint32_t)by, (uint32_t)bz, 16),
+                                      bkey((uint32_t)in, bx, by, bz)});
+                    }
+                });
+                for (auto& v : part) es.insert(es.end(), v.begin(), v.end());
+            } else {
+                std::vector<uint64_t> leaf, act;
+                for (size_t j = j0; j < j1; ++j) {
+                    if (LV[j] != l) continue;
+                    const uint64_t k = bkey((uint32_t)in, X[j] >> 2, Y[j] >> 2, Z[j] >> 2);
+                    leaf.push_back(k);
+                    if (ST[j] == 0) act.push_back(k);
+                }
+                std::sort(act.begin(), act.end());
+                act.erase(std::unique(act.begin(), act.end()), act.end());
+                std::vector<uint64_t> all = leaf;
+                for (uint64_t k : act)
+                    for (int dz = -1; dz <= 1; ++dz)
+                        for (int dy = -1; dy <= 1; ++dy)
+                            for (int dx = -1; dx <= 1; ++dx) {
+                                int32_t q[3] = {bkey_c(k, 0) + dx, bkey_c(k, 1) + dy, bkey_c(k, 2) + dz};
+                                if (wrap_brick(p, nbx, q)) all.push_back(bkey((uint32_t)in, q[0], q[1], q[2]));
+                            }
+                std::sort(all.begin(), all.end());
+                all.erase(std::unique(all.begin(), all.end()), all.end());
+                for (uint64_t k : all)
+                    es.push_back(E{std::binary_search(act.begin(), act.end(), k) ? 0 : 1, (uint32_t)in,
+                                   host_encode((uint32_t)bkey_c(k, 0), (uint32_t)bkey_c(k, 1),
+                                               (uint32_t)bkey_c(k, 2), 16),
+                                   k});
+            }
+        }
+        // order: active first, then by instance, then along the Hilbert curve
         std::sort(es.begin(), es.end(), [](const E& a, const E& b) {
             if (a.act != b.act) return a.act < b.act;
             if (a.inst != b.inst) return a.inst < b.inst;
             return a.h < b.h;
         });
         H.nb = (uint32_t)es.size();
-        H.nactive = (uint32_t)ak.size();
+        H.nactive = 0;
+        while (H.nactive < H.nb && es[H.nactive].act == 0) ++H.nactive;
         H.bcoord.resize((size_t)H.nb * 3);
         H.binst.resize(H.nb);
-        H.bmap.reserve(H.nb * 2);
         H.abeg.assign(NI + 1, 0);
+        for (int in = 0; in < NI; ++in)
+            if (H.grid[in].dense) H.grid[in].idx.assign(H.grid[in].cells(), -1);
         for (uint32_t b = 0; b < H.nb; ++b) {
-            uint64_t k = es[b].k;
+            const uint64_t k = es[b].k;
+            const uint32_t in = bkey_inst(k);
             for (int a = 0; a < 3; ++a) H.bcoord[3 * b + a] = bkey_c(k, a);
-            H.binst[b] = (uint16_t)bkey_inst(k);
-            H.bmap.emplace(k, (int32_t)b);
-            if (b < H.nactive) H.abeg[bkey_inst(k) + 1]++;
+            H.binst[b] = (uint16_t)in;
+            BrickGrid& g = H.grid[in];
+            if (g.dense)
+                g.idx[g.at(bkey_c(k, 0), bkey_c(k, 1), bkey_c(k, 2))] = (int32_t)b;
+            else
+                H.bmap.emplace(k, (int32_t)b);
+            if (b < H.nactive) H.abeg[in + 1]++;
         }
         for (int k = 0; k < NI; ++k) H.abeg[k + 1] += H.abeg[k];
         H.nbr.assign((size_t)H.nb * 27, -1);
-        parallel_for(H.nb, [&](size_t b) {
-            const uint32_t in = H.binst[b];
-            const fl_lbm_params& p = S->inst[in].p;
-            const std::array<int32_t, 3> nbx = {H.iext[in][0] / 4, H.iext[in][1] / 4, H.iext[in][2] / 4};
-            for (int dz = -1; dz <= 1; ++dz)
-                for (int dy = -1; dy <= 1; ++dy)
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        int32_t c[3] = {H.bcoord[3 * b] + dx, H.bcoord[3 * b + 1] + dy, H.bcoord[3 * b + 2] + dz};
-                        if (!wrap_brick(p, nbx, c)) continue;
-                        auto it = H.bmap.find(bkey(in, c[0], c[1], c[2]));
-                        if (it != H.bmap.end()) H.nbr[b * 27 + 13 + dx + 3 * dy + 9 * dz] = it->second;
-                    }
+        parallel_chunks(H.nb, 4096, [&](size_t b0, size_t b1) {
+            for (size_t b = b0; b < b1; ++b) {
+                const uint32_t in = H.binst[b];
+                const fl_lbm_params& p = S->inst[in].p;
+                const BrickGrid& g = H.grid[in];
+                const std::array<int32_t, 3> nbx = {g.n[0], g.n[1], g.n[2]};
+                for (int dz = -1; dz <= 1; ++dz)
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            int32_t c[3] = {H.bcoord[3 * b] + dx, H.bcoord[3 * b + 1] + dy, H.bcoord[3 * b + 2] + dz};
+                            if (!wrap_brick(p, nbx, c)) continue;
+                            H.nbr[b * 27 + 13 + dx + 3 * dy + 9 * dz] = H.brick_at((int)in, c[0], c[1], c[2]);
+                        }
+            }
         });
         H.stype.assign((size_t)H.nb * 64, T_NONE);
         H.needed.assign((size_t)H.nb * 64, 0);
         H.cflags.assign((size_t)H.nb * 64, 0);
-        H.c2fsrc.assign((size_t)H.nb * 64, -1);
         H.bflags.assign(H.nb, 0);
     }
-    // 3. leaf slots
+    // 3. leaf slots (chunk-local counters)
     {
         std::atomic<uint64_t> placed{0}, missing{0};
         std::vector<std::atomic<uint64_t>> nfl((size_t)NL * NI);
         for (auto& a : nfl) a.store(0);
-        parallel_for(n, [&](size_t k) {
-            HostLevel& H = S->h[LV[k]];
-            int64_t s = H.slot_at(linst[k], X[k], Y[k], Z[k]);
-            if (s < 0) { missing.fetch_add(1); return; }
-            uint8_t t = ST[k] == 0 ? T_FLUID : T_SOLID;
-            uint8_t prev = __atomic_exchange_n(&H.stype[(size_t)s], t, __ATOMIC_RELAXED);
-            if (prev == T_NONE) placed.fetch_add(1);
-            if (t == T_FLUID) nfl[(size_t)LV[k] * NI + linst[k]].fetch_add(1);
+        parallel_chunks(n, 1 << 16, [&](size_t j0, size_t j1) {
+            uint64_t pl = 0, mi = 0;
+            std::vector<uint64_t> loc((size_t)NL * NI, 0);
+            for (size_t k = j0; k < j1; ++k) {
+                HostLevel& H = S->h[LV[k]];
+                int64_t s = H.slot_at(linst[k], X[k], Y[k], Z[k]);
+                if (s < 0) { ++mi; continue; }
+                uint8_t t = ST[k] == 0 ? T_FLUID : T_SOLID;
+                uint8_t prev = __atomic_exchange_n(&H.stype[(size_t)s], t, __ATOMIC_RELAXED);
+                if (prev == T_NONE) ++pl;
+                if (t == T_FLUID) ++loc[(size_t)LV[k] * NI + linst[k]];
+            }
+            placed.fetch_add(pl);
+            missing.fetch_add(mi);
+            for (size_t q = 0; q < loc.size(); ++q)
+                if (loc[q]) nfl[q].fetch_add(loc[q]);
         });
         if (missing.load()) return mesh_error("internal: leaf brick missing");
         if (placed.load() != n) return mesh_error("duplicate leaf");
@@ -424,7 +534,7 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
                     const HostLevel& C = S->h[l - 1];
                     int64_t cs = C.slot_at(in, x >> 1, y >> 1, z >> 1);
                     uint8_t ct = cs < 0 ? T_NONE : C.stype[(size_t)cs];
-                    if (ct == T_FLUID) { H.stype[s] = T_C2F; H.c2fsrc[s] = cs; continue; }
+                    if (ct == T_FLUID) { H.stype[s] = T_C2F; continue; }
                     if (ct == T_SOLID) { H.stype[s] = T_WALL; continue; }
                 }
                 if (l + 1 < NL) {
@@ -467,7 +577,7 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
                             uint8_t t2 = s2 < 0 ? T_NONE : H.stype[(size_t)s2];
                             if (regular(t2)) {
                                 code = kLinkOutflow;
-                                __atomic_store_n(&H.needed[(size_t)s2], (uint8_t)1, __ATOMIC_RELAXED);
+                                if (t2 != T_FLUID) __atomic_store_n(&H.needed[(size_t)s2], (uint8_t)1, __ATOMIC_RELAXED);
                             } else if (t2 == T_SOLID || t2 == T_WALL) {
                                 code = kLinkSolid;
                             } else {
@@ -482,7 +592,7 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
                         uint8_t t2 = s2 < 0 ? T_NONE : H.stype[(size_t)s2];
                         if (regular(t2)) {
                             code = kLinkRegular;
-                            __atomic_store_n(&H.needed[(size_t)s2], (uint8_t)1, __ATOMIC_RELAXED);
+                            if (t2 != T_FLUID) __atomic_store_n(&H.needed[(size_t)s2], (uint8_t)1, __ATOMIC_RELAXED);
                         } else if (t2 == T_SOLID || t2 == T_WALL) {
                             code = kLinkSolid;
                             blinks[b].push_back((uint32_t)(s * 32 + i));
@@ -523,7 +633,14 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
         HostLevel& C = S->h[l - 1];
         std::vector<uint32_t> srcs;
         srcs.reserve(H.c2f_ghost.size());
-        for (uint32_t g : H.c2f_ghost) srcs.push_back((uint32_t)H.c2fsrc[g]);
+        for (uint32_t g : H.c2f_ghost) {
+            // the covering coarse leaf of a c2f ghost
+            const size_t b = g / 64;
+            const int loc = (int)(g % 64);
+            const int32_t x = H.bcoord[3 * b] * 4 + (loc & 3), y = H.bcoord[3 * b + 1] * 4 + ((loc >> 2) & 3),
+                          z = H.bcoord[3 * b + 2] * 4 + (loc >> 4);
+            srcs.push_back((uint32_t)C.slot_at(H.binst[b], x >> 1, y >> 1, z >> 1));
+        }
         std::vector<uint32_t> uniq = srcs;
         std::sort(uniq.begin(), uniq.end());
         uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
