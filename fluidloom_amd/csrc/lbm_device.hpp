@@ -71,7 +71,35 @@ struct Store<_Float16> {
     __device__ static inline void store_nt(_Float16* p, size_t i, int q, float v) {
         __builtin_nontemporal_store((_Float16)(v - (float)wq(q)), p + i);
     }
+    // Stochastic rounding of f - w to FP16: r is 13 random bits.  For values in
+    // FP16's normal range, r is added to the 13 FP32 mantissa bits that FP16
+    // drops and those bits are then cleared, so the value rounds up with
+    // probability equal to the dropped fraction (unbiased); smaller values
+    // round to nearest.  Keeps slow updates (below half an FP16 ulp per
+    // step) from being lost, which round-to-nearest does every step.
+    __device__ static inline void store_sr(_Float16* p, size_t i, int q, float v, uint32_t r) {
+        const float d = v - (float)wq(q);
+        const uint32_t b = __float_as_uint(d);
+        uint32_t mag = b & 0x7FFFFFFFu;
+        float e = d;
+        if (mag >= 0x38800000u && mag < 0x477FE000u) {
+            mag = (mag + (r & 0x1FFFu)) & ~0x1FFFu;
+            e = __uint_as_float((b & 0x80000000u) | mag);
+        }
+        __builtin_nontemporal_store((_Float16)e, p + i);
+    }
 };
+
+// 32-bit integer finaliser ("lowbias32", C. Wellons' hash-prospector) used
+// to draw the stochastic-rounding bits from (cell, level, step, direction).
+__device__ inline uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
 
 template <typename R>
 __device__ inline R feq(int i, R rho, R ux, R uy, R uz, R usq) {
@@ -134,6 +162,7 @@ struct LevelDev {
     const float* itau;       // [instance] 1 / tau at this level
     const double* itau_d;
     const InstDev* inst;     // [instance]
+    const uint32_t* tick;    // collide launches of this level so far (stochastic-rounding clock)
     size_t S;                // slot stride = nbricks * 64
     int32_t nactive;         // active bricks (they come first)
     int32_t level;

@@ -18,6 +18,8 @@
 //   k_init, k_brick_sums, k_brick_du, k_inst_reduce, k_link_force,
 //   k_extract, k_inject, k_vorticity, k_remap_*, k_ghost_eq_*: set-up,
 //   deterministic FP64 reductions per instance, queries, regrid transfer.
+#include <type_traits>
+
 #include "fl_internal.hpp"
 #include "lbm_device.hpp"
 #include "lbm_kernels.hpp"
@@ -88,8 +90,20 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
     }
     if (!active) return;
     bgk(f, inv_tau_of<R>(L, inst));
+    if constexpr (std::is_same<T, _Float16>::value) {
+        // rounding bits from the cell's own coordinates, level and step, so an
+        // instance rounds the same way alone and inside any batch
+        const int32_t* c3 = L.bcoord + 3 * b;
+        const uint32_t gx = (uint32_t)(c3[0] * 4 + lx), gy = (uint32_t)(c3[1] * 4 + ly),
+                       gz = (uint32_t)(c3[2] * 4 + lz);
+        const uint32_t seed = mix32(gx * 0x9E3779B1u ^ mix32(gy * 0x85EBCA77u ^ mix32(gz * 0xC2B2AE3Du ^
+                                    ((uint32_t)L.level << 24) ^ (*L.tick * 0x27D4EB2Fu))));
 #pragma unroll
-    for (int i = 0; i < Q; ++i) Store<T>::store_nt(B + (size_t)i * S, slot, i, f[i]);
+        for (int i = 0; i < Q; ++i) Store<T>::store_sr(B + (size_t)i * S, slot, i, f[i], mix32(seed + 0x9E3779B9u * (uint32_t)(i + 1)));
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) Store<T>::store_nt(B + (size_t)i * S, slot, i, f[i]);
+    }
 }
 
 // MODE 0: workgroup g takes brick group g; 1: the same with the XCD-aware
@@ -109,6 +123,10 @@ __global__ __launch_bounds__(64 * BPW) void k_collide(LevelDev L, const T* __res
         const unsigned g = MODE == 1 ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
         collide_brick<T>(L, A, B, (int)(g * BPW) + wave);
     }
+}
+
+__global__ void k_tick(uint32_t* tick) {
+    if (threadIdx.x == 0) *tick += 1u;
 }
 
 template <typename T>
@@ -617,6 +635,11 @@ hipError_t launch_collide(const LevelDev& L, const T* A, T* B, int variant, hipS
         case 9: return collide_variant<T, 8, 0>(L, A, B, s);
         default: return collide_variant<T, 8, 1>(L, A, B, s);
     }
+}
+
+hipError_t launch_tick(uint32_t* tick, hipStream_t s) {
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s, tick);
+    return hipGetLastError();
 }
 
 template <typename T>

@@ -16,6 +16,8 @@ const char* collide_variant_name(int v);
 
 template <typename T>
 hipError_t launch_collide(const LevelDev& L, const T* A, T* B, int variant, hipStream_t s);
+// advances a level's stochastic-rounding clock (one kernel node, graph-safe)
+hipError_t launch_tick(uint32_t* tick, hipStream_t s);
 template <typename T>
 hipError_t launch_capture(const LevelDev& L, const T* A, const uint32_t* slots, uint32_t ncap, void* cap, hipStream_t s);
 template <typename T>

@@ -222,6 +222,8 @@ struct fl_lbm_solver {
     fl::lbm::InstDev* dinst = nullptr;
     std::vector<void*> allocs;
     double* acc = nullptr;  // [ninst * 4] per-instance reduction results
+    uint32_t* ticks = nullptr;  // [24] collide launches per level (FP16 stochastic-rounding clock)
+    bool own_ticks = false;
     int64_t* qslots = nullptr;
     double* qbuf = nullptr;
     size_t qcap = 0;
@@ -248,6 +250,7 @@ struct fl_lbm_solver {
         for (void* a : allocs) (void)hipFree(a);
         if (qslots) (void)hipFree(qslots);
         if (qbuf) (void)hipFree(qbuf);
+        if (own_ticks && ticks) (void)hipFree(ticks);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
     int ninst() const { return (int)inst.size(); }
@@ -744,6 +747,7 @@ fl_status upload_and_init(fl_lbm_solver* S) {
         D.ld.S = slots;
         D.ld.nactive = (int32_t)H.nactive;
         D.ld.level = l;
+        D.ld.tick = S->ticks + l;
         FL_HIP(launch_init<T>(D.ld, static_cast<T*>(D.buf[0]), static_cast<T*>(D.buf[1]), S->stream));
     }
     FL_HIP(hipStreamSynchronize(S->stream));
@@ -768,6 +772,7 @@ fl_status advance(fl_lbm_solver* S, int l) {
     }
     FL_HIP(launch_collide<T>(D.ld, static_cast<const T*>(D.buf[D.cur]), static_cast<T*>(D.buf[1 - D.cur]),
                              S->variant[l], st));
+    if (S->storage == FL_STORE_F16) FL_HIP(launch_tick(S->ticks + l, st));
     D.cur ^= 1;
     if (fine) {
         FL_HIP(launch_capture<T>(D.ld, static_cast<const T*>(D.buf[1 - D.cur]), D.cap_slots, ncap, D.cap[1], st));
@@ -1072,6 +1077,9 @@ fl_status create_solver(const fl_lbm_params* params, int ninst, const size_t* of
         FL_HIP(hipStreamCreate(&S->stream));
         S->own_stream = true;
     }
+    FL_HIP(hipMalloc(reinterpret_cast<void**>(&S->ticks), 24 * sizeof(uint32_t)));
+    S->own_ticks = true;
+    FL_HIP(hipMemsetAsync(S->ticks, 0, 24 * sizeof(uint32_t), S->stream));
     switch (S->storage) {
         case FL_STORE_F32: r = upload_and_init<float>(S.get()); break;
         case FL_STORE_F16: r = upload_and_init<_Float16>(S.get()); break;
@@ -1610,6 +1618,7 @@ fl_status fl_lbm_regrid_batch(fl_lbm_solver* S, const size_t* offsets, const int
     if (r) return r;
     N->stream = S->stream;
     N->hinst = S->hinst;  // keeps the live flags
+    N->ticks = S->ticks;  // and the rounding clock (owned by S)
     switch (S->storage) {
         case FL_STORE_F32: r = upload_and_init<float>(N.get()); break;
         case FL_STORE_F16: r = upload_and_init<_Float16>(N.get()); break;
