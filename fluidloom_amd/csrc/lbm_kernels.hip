@@ -611,12 +611,13 @@ hipError_t collide_variant(const LevelDev& L, const T* A, T* B, hipStream_t s) {
 
 }  // namespace
 
-int collide_variants() { return 11; }
+int collide_variants() { return 13; }
 
 const char* collide_variant_name(int v) {
     static const char* names[] = {"bpw1",     "bpw2",         "bpw4",         "bpw1-xcd",     "bpw2-xcd", "bpw4-xcd",
-                                  "bpw1-persist", "bpw2-persist", "bpw4-persist", "bpw8", "bpw8-xcd"};
-    return v >= 0 && v < 11 ? names[v] : "?";
+                                  "bpw1-persist", "bpw2-persist", "bpw4-persist", "bpw8",     "bpw8-xcd", "bpw16",
+                                  "bpw16-xcd"};
+    return v >= 0 && v < 13 ? names[v] : "?";
 }
 
 template <typename T>
@@ -633,7 +634,9 @@ hipError_t launch_collide(const LevelDev& L, const T* A, T* B, int variant, hipS
         case 7: return collide_variant<T, 2, 2>(L, A, B, s);
         case 8: return collide_variant<T, 4, 2>(L, A, B, s);
         case 9: return collide_variant<T, 8, 0>(L, A, B, s);
-        default: return collide_variant<T, 8, 1>(L, A, B, s);
+        case 10: return collide_variant<T, 8, 1>(L, A, B, s);
+        case 11: return collide_variant<T, 16, 0>(L, A, B, s);
+        default: return collide_variant<T, 16, 1>(L, A, B, s);
     }
 }
 

@@ -47,24 +47,43 @@ __global__ void k_zero_u32(uint32_t* __restrict__ p, size_t n) {
         p[i] = 0u;
 }
 
+// Upfront digit histograms of all passes in one read of the keys.  Each
+// wave counts into its own LDS copy (fewer conflicting LDS atomics), each
+// thread takes kHistItems keys of a block tile with all loads issued first.
+constexpr int kHistItems = 8;
+
 __global__ __launch_bounds__(kThreads) void k_hist(const uint64_t* __restrict__ keys, size_t n,
                                                    int begin_bit, int end_bit, int npass,
                                                    uint32_t* __restrict__ ghist) {
-    __shared__ uint32_t h[8 * 256];
-    for (int i = threadIdx.x; i < npass * 256; i += kThreads) h[i] = 0;
+    __shared__ uint32_t h[kWaves][8 * 256];
+    for (int i = threadIdx.x; i < kWaves * 8 * 256; i += kThreads) (&h[0][0])[i] = 0;
     __syncthreads();
-    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
-        uint64_t k = keys[i];
-        for (int p = 0; p < npass; ++p) {
-            int b = begin_bit + 8 * p;
-            int nb = end_bit - b < 8 ? end_bit - b : 8;
-            unsigned d = (unsigned)((k >> b) & ((1ull << nb) - 1ull));
-            atomicAdd(&h[p * 256 + d], 1u);
+    uint32_t* hw = h[threadIdx.x >> 6];
+    const size_t tile = (size_t)kThreads * kHistItems;
+    for (size_t t0 = (size_t)blockIdx.x * tile; t0 < n; t0 += (size_t)gridDim.x * tile) {
+        uint64_t k[kHistItems];
+#pragma unroll
+        for (int j = 0; j < kHistItems; ++j) {
+            const size_t i = t0 + (size_t)j * kThreads + threadIdx.x;
+            k[j] = i < n ? keys[i] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < kHistItems; ++j) {
+            if (t0 + (size_t)j * kThreads + threadIdx.x >= n) continue;
+            for (int p = 0; p < npass; ++p) {
+                const int b = begin_bit + 8 * p;
+                const int nb = end_bit - b < 8 ? end_bit - b : 8;
+                const unsigned d = (unsigned)((k[j] >> b) & ((1ull << nb) - 1ull));
+                atomicAdd(&hw[p * 256 + d], 1u);
+            }
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < npass * 256; i += kThreads)
-        if (h[i]) atomicAdd(&ghist[i], h[i]);
+    for (int i = threadIdx.x; i < npass * 256; i += kThreads) {
+        uint32_t v = 0;
+        for (int w = 0; w < kWaves; ++w) v += h[w][i];
+        if (v) atomicAdd(&ghist[i], v);
+    }
 }
 
 // Block-wide exclusive scan of one value per thread (256 threads).
@@ -247,8 +266,8 @@ fl_status fl_sort_pairs_u64u32(uint64_t* d_keys, uint32_t* d_vals, uint64_t* d_k
     hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(zero_words, 256) > 4096 ? 4096 : grid_for(zero_words, 256)),
                        dim3(256), 0, st, reinterpret_cast<uint32_t*>(tb), zero_words);
     FL_CHECK_LAUNCH();
-    unsigned hg = grid_for(n, kThreads * 16);
-    if (hg > 1024) hg = 1024;
+    unsigned hg = grid_for(n, kThreads * kHistItems);
+    if (hg > 8192) hg = 8192;
     hipLaunchKernelGGL(k_hist, dim3(hg), dim3(kThreads), 0, st, d_keys, n, begin_bit, end_bit, npass, hist);
     FL_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_scan_hist, dim3(1), dim3(kThreads), 0, st, hist, base, triv, npass, (uint32_t)n);
