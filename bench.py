@@ -30,7 +30,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "AMR D3Q19 MLUPS + HBM GB/s; batched solves/s & p99 latency vs plain"
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_UPDATE = {"f32": 152, "f16": 76, "f64": 304}
+BYTES_PER_UPDATE = {"f32": 152, "f16": 76, "f16c": 76, "f64": 304}
 
 
 def log(msg):
@@ -96,7 +96,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "f64"])
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "f16c", "f64"])
     ap.add_argument("--scale", type=int, default=1, help="divide the base grid and radii by this")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--regrid-every", type=int, default=100)

@@ -25,6 +25,7 @@ FL_BC_PERIODIC = 2
 FL_STORE_F32 = 0
 FL_STORE_F16 = 1
 FL_STORE_F64 = 2
+FL_STORE_F16C = 3
 
 vp = C.c_void_p
 sz = C.c_size_t

@@ -112,7 +112,8 @@ def plain_rate(specs, mesh_list, steps, sample):
     return upd / t / 1e6, upd, t
 
 
-def run(instances=256, steps=2000, regrid_every=0, plain_sample=32, plain_steps=200, seed=42, nmax=96):
+def run(instances=256, steps=2000, regrid_every=0, plain_sample=32, plain_steps=200, seed=42, nmax=96,
+        forms=("f32", "f16", "f16c")):
     check(lib().fl_init(0))
     specs = instance_specs(instances, seed, nmax=nmax)
     t0 = time.perf_counter()
@@ -120,7 +121,7 @@ def run(instances=256, steps=2000, regrid_every=0, plain_sample=32, plain_steps=
     log(f"{instances} meshes built in {time.perf_counter() - t0:.1f} s; "
         f"{sum(len(m['x']) for m in mesh_list)} leaves")
     batches = {}
-    for st in ("f64", "f32", "f16"):
+    for st in ("f64",) + tuple(forms):
         t1 = time.perf_counter()
         b = LbmBatch([cavity_params(s, st) for s in specs], mesh_list)
         if st != "f64":
@@ -130,11 +131,13 @@ def run(instances=256, steps=2000, regrid_every=0, plain_sample=32, plain_steps=
         batches[st] = b
         log(f"{st} batch built in {time.perf_counter() - t1:.1f} s, "
             f"{b.stats().device_bytes / 2**30:.2f} GiB")
-    upd_step = int(batches["f32"].stats().updates_per_step)
+    upd_step = int(batches["f64"].stats().updates_per_step)
     m0 = {st: b.totals_batch()[0] for st, b in batches.items()}
     times = {st: 0.0 for st in batches}
     done = 0
     regrids = 0
+    upd_done = 0
+    regrid_s = 0.0
     while done < steps:
         n = steps - done
         if regrid_every:
@@ -142,31 +145,36 @@ def run(instances=256, steps=2000, regrid_every=0, plain_sample=32, plain_steps=
         for st, b in batches.items():
             times[st] += timed_steps(b, n)
         done += n
+        upd_done += upd_step * n
         if regrid_every and done % regrid_every == 0 and done < steps:
+            r0 = time.perf_counter()
             new = regrid_meshes(batches["f64"], specs)
             for b in batches.values():
                 b.regrid_batch(new)
+            regrid_s += time.perf_counter() - r0
             regrids += 1
-            upd_step = int(batches["f32"].stats().updates_per_step)
-        log(f"step {done}/{steps}: " + ", ".join(f"{st} {upd_step * done / max(t, 1e-9) / 1e6:.0f} MLUPS"
+            upd_step = int(batches["f64"].stats().updates_per_step)
+        log(f"step {done}/{steps}: " + ", ".join(f"{st} {upd_done / max(t, 1e-9) / 1e6:.0f} MLUPS"
                                                  for st, t in times.items()))
     res = {}
     errs = {}
-    for st in ("f32", "f16"):
+    for st in forms:
         e = velocity_error(batches[st], batches["f64"])
         errs[st] = e
         m = batches[st].totals_batch()[0]
         drift = np.abs(m - m0[st]) / m0[st]
-        res[st] = dict(mlups=round(upd_step * steps / times[st] / 1e6, 1), seconds=round(times[st], 3),
+        res[st] = dict(mlups=round(upd_done / times[st] / 1e6, 1), seconds=round(times[st], 3),
                        solves_per_s=round(instances / times[st], 3),
                        vel_l2_max=float(e.max()), vel_l2_median=float(np.median(e)),
                        mass_drift_max=float(drift.max()))
-    worst = np.argsort(-errs["f16"])[:5]
-    res["f16_worst"] = [dict(instance=int(k), **specs[k], vel_l2_f16=float(errs["f16"][k]),
-                             vel_l2_f32=float(errs["f32"][k])) for k in worst]
-    res["f16_frac_below_1e-3"] = float((errs["f16"] <= 1e-3).mean())
+    for st in forms:
+        if st == "f32":
+            continue
+        worst = np.argsort(-errs[st])[:5]
+        res[f"{st}_worst"] = [dict(instance=int(k), **specs[k], vel_l2=float(errs[st][k])) for k in worst]
+        res[f"{st}_frac_below_1e-3"] = float((errs[st] <= 1e-3).mean())
     m64 = batches["f64"].totals_batch()[0]
-    res["f64"] = dict(mlups=round(upd_step * steps / times["f64"] / 1e6, 1), seconds=round(times["f64"], 3),
+    res["f64"] = dict(mlups=round(upd_done / times["f64"] / 1e6, 1), seconds=round(times["f64"], 3),
                       mass_drift_max=float((np.abs(m64 - m0["f64"]) / m0["f64"]).max()))
     for b in batches.values():
         b.close()
@@ -174,11 +182,11 @@ def run(instances=256, steps=2000, regrid_every=0, plain_sample=32, plain_steps=
     out = dict(workload=f"{instances} lid-driven cavities, base 16^3..{nmax}^3 log-uniform, lid U(0.02,0.10), "
                         f"tau U(0.55,0.9), 0-2 levels, {steps} steps" +
                         (f", regrid every {regrid_every}" if regrid_every else ""),
-               updates_per_step=upd_step, regrids=regrids, results=res,
+               updates_per_step=upd_step, regrids=regrids, regrid_seconds_all_forms=round(regrid_s, 2),
+               results=res,
                plain_f32=dict(mlups=round(plain, 1), sample=f"first {min(plain_sample, instances)} instances x "
                                                             f"{plain_steps} steps, single-instance eager"),
-               f16_over_plain=round(res["f16"]["mlups"] / plain, 2),
-               f32_over_plain=round(res["f32"]["mlups"] / plain, 2))
+               over_plain={st: round(res[st]["mlups"] / plain, 2) for st in forms})
     return out
 
 
@@ -191,9 +199,10 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=200)
     ap.add_argument("--nmax", type=int, default=96)
     ap.add_argument("--seed", type=int, default=42)
+    ap.add_argument("--forms", default="f32,f16,f16c", help="storage forms measured against FP64")
     a = ap.parse_args()
-    print(json.dumps(run(a.instances, a.steps, a.regrid_every, a.plain_sample, a.plain_steps, a.seed, a.nmax)),
-          flush=True)
+    print(json.dumps(run(a.instances, a.steps, a.regrid_every, a.plain_sample, a.plain_steps, a.seed, a.nmax,
+                         tuple(a.forms.split(",")))), flush=True)
 
 
 if __name__ == "__main__":

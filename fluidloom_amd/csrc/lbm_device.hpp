@@ -90,6 +90,58 @@ struct Store<_Float16> {
     }
 };
 
+// FP16C: 16-bit storage of x = (f - w_q) / w_q with one sign bit, a 4-bit
+// exponent (bias 14) and an 11-bit mantissa: one more mantissa bit than
+// IEEE binary16 for |x| < 4 (values beyond saturate), subnormal spacing 2^-24
+// below 2^-13.  Low-Mach populations stay well inside |x| < 1.
+struct fp16c {
+    uint16_t bits;
+};
+
+__device__ inline float fp16c_decode(uint16_t c) {
+    const uint32_t m = c & 0x7FFFu;
+    const float a = m >= 0x800u ? __uint_as_float((((m >> 11) + 113u) << 23) | ((m & 0x7FFu) << 12))
+                                : (float)m * 5.9604644775390625e-08f;
+    return __uint_as_float(__float_as_uint(a) | ((uint32_t)(c & 0x8000u) << 16));
+}
+
+// rnd: 12 random bits for stochastic rounding, or round to nearest even when
+// stochastic is false
+__device__ inline uint16_t fp16c_encode(float x, uint32_t rnd, bool stochastic) {
+    const uint32_t b = __float_as_uint(x);
+    const uint32_t sign = (b >> 16) & 0x8000u;
+    const uint32_t mag = b & 0x7FFFFFFFu;
+    uint32_t code;
+    if (mag >= 0x39000000u) {  // normal range (|x| >= 2^-13)
+        const uint32_t add = stochastic ? (rnd & 0xFFFu) : 0x7FFu + ((mag >> 12) & 1u);
+        const uint32_t m2 = mag + add;
+        code = (mag >= 0x7F800000u || m2 >= 0x40800000u) ? 0x7FFFu : (m2 - (113u << 23)) >> 12;
+    } else {
+        const float a = __uint_as_float(mag) * 16777216.0f;  // 2^24
+        code = stochastic ? (uint32_t)(a + (float)(rnd & 0xFFFu) * (1.0f / 4096.0f)) : (uint32_t)rintf(a);
+    }
+    return (uint16_t)(sign | code);
+}
+
+template <>
+struct Store<fp16c> {
+    using R = float;
+    __device__ static inline float load(const fp16c* p, size_t i, int q) {
+        const float w = (float)wq(q);
+        return fmaf(fp16c_decode(p[i].bits), w, w);
+    }
+    __device__ static inline uint16_t enc(float v, int q, uint32_t r, bool sr) {
+        return fp16c_encode((v - (float)wq(q)) * (float)(1.0 / wq(q)), r, sr);
+    }
+    __device__ static inline void store(fp16c* p, size_t i, int q, float v) { p[i].bits = enc(v, q, 0, false); }
+    __device__ static inline void store_nt(fp16c* p, size_t i, int q, float v) {
+        __builtin_nontemporal_store(enc(v, q, 0, false), &p[i].bits);
+    }
+    __device__ static inline void store_sr(fp16c* p, size_t i, int q, float v, uint32_t r) {
+        __builtin_nontemporal_store(enc(v, q, r, true), &p[i].bits);
+    }
+};
+
 // 32-bit integer finaliser ("lowbias32", C. Wellons' hash-prospector) used
 // to draw the stochastic-rounding bits from (cell, level, step, direction).
 __device__ inline uint32_t mix32(uint32_t x) {

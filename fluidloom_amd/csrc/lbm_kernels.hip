@@ -90,7 +90,7 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
     }
     if (!active) return;
     bgk(f, inv_tau_of<R>(L, inst));
-    if constexpr (std::is_same<T, _Float16>::value) {
+    if constexpr (std::is_same<T, _Float16>::value || std::is_same<T, fp16c>::value) {
         // rounding bits from the cell's own coordinates, level and step, so an
         // instance rounds the same way alone and inside any batch
         const int32_t* c3 = L.bcoord + 3 * b;
@@ -783,6 +783,7 @@ hipError_t launch_ghost_eq(const uint32_t* c2f_ghost, const uint32_t* c2f_cap, c
 FL_INST(float)
 FL_INST(double)
 FL_INST(_Float16)
+FL_INST(fp16c)
 
 }  // namespace lbm
 }  // namespace fl

@@ -203,7 +203,9 @@ struct DevLevel {
     LevelDev ld{};
 };
 
-size_t store_bytes(int storage) { return storage == FL_STORE_F64 ? 8 : (storage == FL_STORE_F16 ? 2 : 4); }
+size_t store_bytes(int storage) {
+    return storage == FL_STORE_F64 ? 8 : ((storage == FL_STORE_F16 || storage == FL_STORE_F16C) ? 2 : 4);
+}
 size_t compute_bytes(int storage) { return storage == FL_STORE_F64 ? 8 : 4; }
 
 }  // namespace
@@ -772,7 +774,7 @@ fl_status advance(fl_lbm_solver* S, int l) {
     }
     FL_HIP(launch_collide<T>(D.ld, static_cast<const T*>(D.buf[D.cur]), static_cast<T*>(D.buf[1 - D.cur]),
                              S->variant[l], st));
-    if (S->storage == FL_STORE_F16) FL_HIP(launch_tick(S->ticks + l, st));
+    if (S->storage == FL_STORE_F16 || S->storage == FL_STORE_F16C) FL_HIP(launch_tick(S->ticks + l, st));
     D.cur ^= 1;
     if (fine) {
         FL_HIP(launch_capture<T>(D.ld, static_cast<const T*>(D.buf[1 - D.cur]), D.cap_slots, ncap, D.cap[1], st));
@@ -792,6 +794,7 @@ fl_status advance_any(fl_lbm_solver* S) {
     switch (S->storage) {
         case FL_STORE_F32: return advance<float>(S, 0);
         case FL_STORE_F16: return advance<_Float16>(S, 0);
+        case FL_STORE_F16C: return advance<fp16c>(S, 0);
         default: return advance<double>(S, 0);
     }
 }
@@ -1003,7 +1006,7 @@ fl_status check_params(const fl_lbm_params* p) {
     FL_REQUIRE(((int64_t)std::max(p->nx0, std::max(p->ny0, p->nz0)) << p->max_level) <= (int64_t)1 << 18,
                "finest-level extent must not exceed 2^18 cells");
     FL_REQUIRE(p->tau0 > 0.5, "tau0 must exceed 1/2");
-    FL_REQUIRE(p->storage >= 0 && p->storage <= 2, "unknown storage type");
+    FL_REQUIRE(p->storage >= 0 && p->storage <= 3, "unknown storage type");
     for (int f = 0; f < 6; ++f) FL_REQUIRE(p->bc[f] >= 0 && p->bc[f] <= 2, "unknown boundary type");
     for (int a = 0; a < 3; ++a)
         FL_REQUIRE((p->bc[2 * a] == FL_BC_PERIODIC) == (p->bc[2 * a + 1] == FL_BC_PERIODIC),
@@ -1083,6 +1086,7 @@ fl_status create_solver(const fl_lbm_params* params, int ninst, const size_t* of
     switch (S->storage) {
         case FL_STORE_F32: r = upload_and_init<float>(S.get()); break;
         case FL_STORE_F16: r = upload_and_init<_Float16>(S.get()); break;
+        case FL_STORE_F16C: r = upload_and_init<fp16c>(S.get()); break;
         default: r = upload_and_init<double>(S.get()); break;
     }
     if (r) return r;
@@ -1283,6 +1287,7 @@ fl_status time_any(fl_lbm_solver* S, int level, int variant, int iters, float* m
     switch (S->storage) {
         case FL_STORE_F32: return time_collide<float>(S, level, variant, iters, ms);
         case FL_STORE_F16: return time_collide<_Float16>(S, level, variant, iters, ms);
+        case FL_STORE_F16C: return time_collide<fp16c>(S, level, variant, iters, ms);
         default: return time_collide<double>(S, level, variant, iters, ms);
     }
 }
@@ -1391,6 +1396,7 @@ fl_status fl_lbm_get_f_inst(fl_lbm_solver* S, int inst, const int32_t* x, const 
     switch (S->storage) {
         case FL_STORE_F32: return get_f<float>(S, inst, x, y, z, lv, n, f);
         case FL_STORE_F16: return get_f<_Float16>(S, inst, x, y, z, lv, n, f);
+        case FL_STORE_F16C: return get_f<fp16c>(S, inst, x, y, z, lv, n, f);
         default: return get_f<double>(S, inst, x, y, z, lv, n, f);
     }
 }
@@ -1407,6 +1413,7 @@ fl_status fl_lbm_set_f_inst(fl_lbm_solver* S, int inst, const int32_t* x, const 
     switch (S->storage) {
         case FL_STORE_F32: return set_f<float>(S, inst, x, y, z, lv, n, f);
         case FL_STORE_F16: return set_f<_Float16>(S, inst, x, y, z, lv, n, f);
+        case FL_STORE_F16C: return set_f<fp16c>(S, inst, x, y, z, lv, n, f);
         default: return set_f<double>(S, inst, x, y, z, lv, n, f);
     }
 }
@@ -1451,6 +1458,7 @@ fl_status fl_lbm_totals_batch(fl_lbm_solver* S, double* mass, double* mom) {
     switch (S->storage) {
         case FL_STORE_F32: r = totals<float>(S, a.data()); break;
         case FL_STORE_F16: r = totals<_Float16>(S, a.data()); break;
+        case FL_STORE_F16C: r = totals<fp16c>(S, a.data()); break;
         default: r = totals<double>(S, a.data()); break;
     }
     if (r) return r;
@@ -1480,6 +1488,7 @@ fl_status fl_lbm_force_batch(fl_lbm_solver* S, double* F) {
     switch (S->storage) {
         case FL_STORE_F32: return force<float>(S, F);
         case FL_STORE_F16: return force<_Float16>(S, F);
+        case FL_STORE_F16C: return force<fp16c>(S, F);
         default: return force<double>(S, F);
     }
 }
@@ -1500,6 +1509,7 @@ fl_status fl_lbm_velocity_change(fl_lbm_solver* S, double* du) {
     switch (S->storage) {
         case FL_STORE_F32: return velocity_change<float>(S, du);
         case FL_STORE_F16: return velocity_change<_Float16>(S, du);
+        case FL_STORE_F16C: return velocity_change<fp16c>(S, du);
         default: return velocity_change<double>(S, du);
     }
 }
@@ -1599,6 +1609,7 @@ fl_status fl_lbm_vorticity_inst(fl_lbm_solver* S, int inst, const int32_t* x, co
     switch (S->storage) {
         case FL_STORE_F32: return vorticity<float>(S, inst, x, y, z, lv, n, out);
         case FL_STORE_F16: return vorticity<_Float16>(S, inst, x, y, z, lv, n, out);
+        case FL_STORE_F16C: return vorticity<fp16c>(S, inst, x, y, z, lv, n, out);
         default: return vorticity<double>(S, inst, x, y, z, lv, n, out);
     }
 }
@@ -1622,12 +1633,14 @@ fl_status fl_lbm_regrid_batch(fl_lbm_solver* S, const size_t* offsets, const int
     switch (S->storage) {
         case FL_STORE_F32: r = upload_and_init<float>(N.get()); break;
         case FL_STORE_F16: r = upload_and_init<_Float16>(N.get()); break;
+        case FL_STORE_F16C: r = upload_and_init<fp16c>(N.get()); break;
         default: r = upload_and_init<double>(N.get()); break;
     }
     if (r) return r;
     switch (S->storage) {
         case FL_STORE_F32: r = remap<float>(S, N.get()); break;
         case FL_STORE_F16: r = remap<_Float16>(S, N.get()); break;
+        case FL_STORE_F16C: r = remap<fp16c>(S, N.get()); break;
         default: r = remap<double>(S, N.get()); break;
     }
     if (r) return r;

@@ -13,10 +13,10 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (FL_BC_OUTFLOW, FL_BC_PERIODIC, FL_BC_WALL, FL_STORE_F16, FL_STORE_F32,
+from ._lib import (FL_BC_OUTFLOW, FL_BC_PERIODIC, FL_BC_WALL, FL_STORE_F16, FL_STORE_F16C, FL_STORE_F32,
                    FL_STORE_F64, LbmParams, LbmStats, check, lib)
 
-STORAGE = {"f32": FL_STORE_F32, "f16": FL_STORE_F16, "f64": FL_STORE_F64}
+STORAGE = {"f32": FL_STORE_F32, "f16": FL_STORE_F16, "f64": FL_STORE_F64, "f16c": FL_STORE_F16C}
 BC = {"wall": FL_BC_WALL, "outflow": FL_BC_OUTFLOW, "periodic": FL_BC_PERIODIC}
 
 

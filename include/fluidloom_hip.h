@@ -235,7 +235,11 @@ fl_status fl_ghost_fill_f2c(const float* d_fine, const uint32_t* d_child8, size_
 
 /* ---- a15/a16: AMR D3Q19 lattice-Boltzmann solver ------------------------ */
 enum { FL_BC_WALL = 0, FL_BC_OUTFLOW = 1, FL_BC_PERIODIC = 2 };
-enum { FL_STORE_F32 = 0, FL_STORE_F16 = 1, FL_STORE_F64 = 2 };
+/* F32: float; F16: IEEE binary16 holding f - w_q; F64: double (the GPU FP64
+ * oracle); F16C: (f - w_q) / w_q in a 16-bit format with a 4-bit exponent
+ * and an 11-bit mantissa (|x| < 4).  Both 16-bit forms round the
+ * collide-stream output stochastically. */
+enum { FL_STORE_F32 = 0, FL_STORE_F16 = 1, FL_STORE_F64 = 2, FL_STORE_F16C = 3 };
 
 typedef struct {
     int32_t nx0, ny0, nz0;  /* level-0 grid */

@@ -30,7 +30,7 @@ def batch_of(oracle, names, storage, graph=False):
     return css, LbmBatch([params_of(cs, storage) for cs in css], [mesh_of(cs) for cs in css], graph=graph)
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16", "f64"])
+@pytest.mark.parametrize("storage", ["f32", "f16", "f16c", "f64"])
 def test_batch_equals_single(fl, oracle, storage):
     names = CASES + ["cavity2", "sphere3"]
     css, b = batch_of(oracle, names, storage)

@@ -57,7 +57,7 @@ def test_f32_and_f16_fields_after_steps(fl, oracle, name):
     o = oracle_solver(oracle, cs)
     o.step(60)
     ro, uo = o.moments()
-    for storage, tol_rho, tol_u in [("f32", 1e-4, 1e-4), ("f16", 1e-3, 1e-3)]:
+    for storage, tol_rho, tol_u in [("f32", 1e-4, 1e-4), ("f16", 1e-3, 1e-3), ("f16c", 1e-3, 1e-3)]:
         g = gpu_solver(cs, storage)
         g.step(60)
         rg, ug = g.moments()
@@ -108,3 +108,25 @@ def test_cavity_with_regrid_matches_oracle(fl, oracle):
     fg = g.solver.get_f(o.s.fx, o.s.fy, o.s.fz, o.s.flv)
     assert np.abs(fg - fo).max() / np.abs(fo).max() < 1e-10
     assert (g.solver.level == 1).any()
+
+
+def test_f16c_codec_matches_format(fl, oracle):
+    """Device FP16C codec == the numpy restatement of the format: values on
+    the format's grid come back on the same grid point, others round to the
+    nearest grid value (set_f stores with round-to-nearest).  |x| >= 2^-8 so
+    that the FP32 population itself carries x to well below the format's
+    step (2^-11 relative); rtol 1e-4 separates neighbouring grid values."""
+    from tests import fp16c_ref as F
+    cs = case("periodic", oracle)
+    g = gpu_solver(cs, "f16c")
+    n = len(g.fx)
+    w = np.asarray(oracle.lattice()[1], np.float64)
+    rng = np.random.default_rng(5)
+    e = rng.integers(6, 15, (n, 19)).astype(np.uint32)          # exponents 2^-8 .. 2^0
+    codes = (rng.integers(0, 2, (n, 19)).astype(np.uint32) << 15) | (e << 11) | rng.integers(0, 2048, (n, 19)).astype(np.uint32)
+    x = F.decode(codes)
+    g.set_f(w * (1.0 + x))
+    assert np.allclose(g.get_f() / w - 1.0, x, rtol=1e-4, atol=0)
+    xr = rng.choice([-1.0, 1.0], (n, 19)) * 2.0 ** rng.uniform(-8, -1, (n, 19))
+    g.set_f(w * (1.0 + xr))
+    assert np.allclose(g.get_f() / w - 1.0, F.decode(F.encode_rne(xr)), rtol=1e-4, atol=0)
