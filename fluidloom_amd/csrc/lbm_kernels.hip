@@ -6,12 +6,13 @@
 //               neighbour row is held in scalar registers, so a regular
 //               pull costs one population load plus a few lane selects;
 //               cells of "simple" bricks (all 64 active, no wall links)
-//               never read their link word.  Workgroups are remapped so that
-//               each XCD processes one contiguous Hilbert-ordered range of
-//               bricks, keeping cross-brick re-reads inside one L2.
-//               Variants: BPW = bricks (waves) per workgroup, with or
-//               without the XCD remap, or persistent (chosen per level by
-//               the solver's autotuner).
+//               never read their link word.  Variants: BPW = 1..16 bricks
+//               (waves) per workgroup, with or without an XCD-aware remap
+//               (each XCD takes one contiguous Hilbert range of bricks), or
+//               persistent; the solver's autotuner picks one per level.
+//               16-bit storages (FP16 f - w, FP16C) round the output
+//               stochastically, with bits drawn from the cell's coordinates,
+//               level and the level's step clock.
 //   k_capture   pre-collision state of coarse cells that feed c2f ghosts
 //   k_fill_c2f  coarse -> fine ghost populations (f_neq rescaled)
 //   k_fill_f2c  fine -> coarse ghost populations (8-child average)

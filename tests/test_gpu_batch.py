@@ -141,3 +141,31 @@ def test_velocity_change(fl, oracle):
         host = np.abs(u_now.astype(np.float32).astype(np.float64) - u_prev[k].astype(np.float32).astype(np.float64))
         assert abs(du[k] - host.max()) <= 1e-6 * max(host.max(), 1e-12) + 1e-9, k
     assert np.all(du > 0)
+
+
+def test_batched_regrid_equals_single_regrids(fl, oracle):
+    """Two cavities of different size, lid and tau stepped as one batch with a
+    vorticity-driven regrid of both (fl_lbm_regrid_batch): every instance
+    ends bit-identical to the same cavity run alone with fl_lbm_regrid."""
+    from fluidloom_amd import batched
+    from fluidloom_amd.solver import LbmBatch
+    specs = [dict(n0=16, lid=0.08, tau=0.6, levels=1), dict(n0=12, lid=0.05, tau=0.7, levels=1)]
+    meshes = batched.build_meshes([dict(s, levels=0) for s in specs])
+    for storage in ("f32", "f16"):
+        b = LbmBatch([batched.cavity_params(s, storage) for s in specs], meshes)
+        singles = [LbmBatch([batched.cavity_params(s, storage)], [m]) for s, m in zip(specs, meshes)]
+        for rnd in range(2):
+            b.step(30)
+            for s in singles:
+                s.step(30)
+            # flags from the batch, applied to both forms
+            new = batched.regrid_meshes(b, specs)
+            b.regrid_batch(new)
+            for k, s in enumerate(singles):
+                s.regrid_batch([new[k]])
+        b.step(10)
+        for s in singles:
+            s.step(10)
+        for k, s in enumerate(singles):
+            assert np.array_equal(b.get_f(k), s.get_f(0)), (storage, k)
+            assert (np.asarray(b.meshes[k]["level"]) == 1).any()
