@@ -190,6 +190,11 @@ __device__ inline void bgk(R* f, R inv_tau) {
 // wall term of the face), 3 outflow copy from the position clamped into the
 // domain.
 constexpr uint64_t kActiveBit = 1ull << 63;
+
+// c2f capture indices per ghost: the covering coarse cell, then its coarse
+// fluid neighbours along -x, +x, -y, +y, -z, +z (kNoCap when absent)
+constexpr int kC2FCaps = 7;
+constexpr uint32_t kNoCap = 0xFFFFFFFFu;
 enum : unsigned { kLinkRegular = 0, kLinkSolid = 1, kLinkWall = 2, kLinkOutflow = 3 };
 
 // Per-instance constants (one record per instance of a batch).

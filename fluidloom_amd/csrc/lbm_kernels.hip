@@ -14,7 +14,9 @@
 //               stochastically, with bits drawn from the cell's coordinates,
 //               level and the level's step clock.
 //   k_capture   pre-collision state of coarse cells that feed c2f ghosts
-//   k_fill_c2f  coarse -> fine ghost populations (f_neq rescaled)
+//   k_fill_c2f  coarse -> fine ghost populations: density and velocity
+//               interpolated linearly from the covering coarse cell and its
+//               coarse fluid neighbours, f_neq rescaled
 //   k_fill_f2c  fine -> coarse ghost populations (8-child average)
 //   k_init, k_brick_sums, k_brick_du, k_inst_reduce, k_link_force,
 //   k_extract, k_inject, k_vorticity, k_remap_*, k_ghost_eq_*: set-up,
@@ -153,6 +155,16 @@ __device__ inline double f2c_factor(double tau0, int l) {
     return (1.0 - 1.0 / tc) * (2.0 * tc / tf);
 }
 
+// Time-interpolated captured state of one coarse cell and its moments
+// (rho, ux, uy, uz).
+template <typename R>
+__device__ inline void cap_state(const R* __restrict__ cap0, const R* __restrict__ cap1, uint32_t ncap, uint32_t c,
+                                 R alpha, R* f, R* m) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = ((R)1 - alpha) * cap0[(size_t)i * ncap + c] + alpha * cap1[(size_t)i * ncap + c];
+    moments(f, m[0], m[1], m[2], m[3]);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kWG) void k_fill_c2f(LevelDev Lf, const uint32_t* __restrict__ ghost,
                                                   const uint32_t* __restrict__ capidx, uint32_t n,
@@ -165,18 +177,35 @@ __global__ __launch_bounds__(kWG) void k_fill_c2f(LevelDev Lf, const uint32_t* _
         const InstDev& I = Lf.inst[Lf.binst[g >> 6]];
         if (!I.live) continue;
         const R fac = (R)c2f_factor(I.tau0, Lf.level);
-        const uint32_t c = capidx[k];
-        R f[Q];
+        // capture indices: covering coarse cell P, then its coarse fluid
+        // neighbours -x, +x, -y, +y, -z, +z (kNoCap where there is none)
+        const uint32_t* ci = capidx + (size_t)kC2FCaps * k;
+        R f[Q], mp[4];
+        cap_state(cap0, cap1, ncap, ci[0], alpha, f, mp);
+        R phi[4] = {mp[0], mp[1], mp[2], mp[3]};
+        const int loc = (int)(g & 63);
+        const int par[3] = {loc & 1, (loc >> 2) & 1, (loc >> 4) & 1};  // fine-cell parity per axis
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t nm = ci[1 + 2 * a], nq = ci[2 + 2 * a];
+            R fn[Q], mm[4] = {0, 0, 0, 0}, mq[4] = {0, 0, 0, 0};
+            if (nm != kNoCap) cap_state(cap0, cap1, ncap, nm, alpha, fn, mm);
+            if (nq != kNoCap) cap_state(cap0, cap1, ncap, nq, alpha, fn, mq);
+            const R dl = par[a] ? (R)0.25 : (R)-0.25;
 #pragma unroll
-        for (int i = 0; i < Q; ++i)
-            f[i] = ((R)1 - alpha) * cap0[(size_t)i * ncap + c] + alpha * cap1[(size_t)i * ncap + c];
-        R rho, ux, uy, uz;
-        moments(f, rho, ux, uy, uz);
-        const R usq = ux * ux + uy * uy + uz * uz;
+            for (int j = 0; j < 4; ++j) {
+                R gr = 0;
+                if (nm != kNoCap && nq != kNoCap) gr = (R)0.5 * (mq[j] - mm[j]);
+                else if (nq != kNoCap) gr = mq[j] - mp[j];
+                else if (nm != kNoCap) gr = mp[j] - mm[j];
+                phi[j] += dl * gr;
+            }
+        }
+        const R usq = phi[1] * phi[1] + phi[2] * phi[2] + phi[3] * phi[3];
+        const R usp = mp[1] * mp[1] + mp[2] * mp[2] + mp[3] * mp[3];
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            const R e = feq(i, rho, ux, uy, uz, usq);
-            Store<T>::store(A, (size_t)i * Lf.S + g, i, e + fac * (f[i] - e));
+            const R e = feq(i, phi[0], phi[1], phi[2], phi[3], usq);
+            Store<T>::store(A, (size_t)i * Lf.S + g, i, e + fac * (f[i] - feq(i, mp[0], mp[1], mp[2], mp[3], usp)));
         }
     }
 }
@@ -525,7 +554,7 @@ __global__ __launch_bounds__(kWG) void k_ghost_eq_c2f(const uint32_t* __restrict
                                                       T* __restrict__ B, size_t S) {
     using R = typename Store<T>::R;
     for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < n; k += gridDim.x * kWG) {
-        const size_t cs = cap_slots[capidx[k]];
+        const size_t cs = cap_slots[capidx[(size_t)kC2FCaps * k]];
         R f[Q];
 #pragma unroll
         for (int i = 0; i < Q; ++i) f[i] = Store<T>::load(Ac, (size_t)i * Sc + cs, i);

@@ -633,26 +633,52 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
             if (H.stype[s] == T_F2C) H.f2c_ghost.push_back((uint32_t)s);
         }
     }
+    // c2f sources: the covering coarse leaf of every ghost and that leaf's
+    // coarse fluid neighbours along the three axes (across periodic faces),
+    // all captured once per coarse step
     for (int l = 1; l < NL; ++l) {
         HostLevel& H = S->h[l];
         HostLevel& C = S->h[l - 1];
-        std::vector<uint32_t> srcs;
-        srcs.reserve(H.c2f_ghost.size());
-        for (uint32_t g : H.c2f_ghost) {
-            // the covering coarse leaf of a c2f ghost
-            const size_t b = g / 64;
-            const int loc = (int)(g % 64);
-            const int32_t x = H.bcoord[3 * b] * 4 + (loc & 3), y = H.bcoord[3 * b + 1] * 4 + ((loc >> 2) & 3),
-                          z = H.bcoord[3 * b + 2] * 4 + (loc >> 4);
-            srcs.push_back((uint32_t)C.slot_at(H.binst[b], x >> 1, y >> 1, z >> 1));
-        }
-        std::vector<uint32_t> uniq = srcs;
+        const size_t ng = H.c2f_ghost.size();
+        std::vector<int64_t> src(ng * kC2FCaps, -1);
+        parallel_chunks(ng, 4096, [&](size_t k0, size_t k1) {
+            for (size_t k = k0; k < k1; ++k) {
+                const uint32_t g = H.c2f_ghost[k];
+                const size_t b = g / 64;
+                const int loc = (int)(g % 64);
+                const int in = H.binst[b];
+                const fl_lbm_params& p = S->inst[in].p;
+                const int32_t pc[3] = {(H.bcoord[3 * b] * 4 + (loc & 3)) >> 1,
+                                       (H.bcoord[3 * b + 1] * 4 + ((loc >> 2) & 3)) >> 1,
+                                       (H.bcoord[3 * b + 2] * 4 + (loc >> 4)) >> 1};
+                src[k * kC2FCaps] = C.slot_at(in, pc[0], pc[1], pc[2]);
+                for (int a = 0; a < 3; ++a)
+                    for (int d = 0; d < 2; ++d) {
+                        int32_t q[3] = {pc[0], pc[1], pc[2]};
+                        q[a] += d ? 1 : -1;
+                        const int32_t e = C.iext[in][a];
+                        if (q[a] < 0 || q[a] >= e) {
+                            if (p.bc[2 * a + d] != FL_BC_PERIODIC) continue;
+                            q[a] = (q[a] + e) % e;
+                        }
+                        const int64_t s2 = C.slot_at(in, q[0], q[1], q[2]);
+                        if (s2 >= 0 && C.stype[(size_t)s2] == T_FLUID) src[k * kC2FCaps + 1 + 2 * a + d] = s2;
+                    }
+            }
+        });
+        for (size_t k = 0; k < ng; ++k)
+            if (src[k * kC2FCaps] < 0) return mesh_error("c2f ghost with no covering coarse leaf");
+        std::vector<uint32_t> uniq;
+        uniq.reserve(src.size());
+        for (int64_t v : src)
+            if (v >= 0) uniq.push_back((uint32_t)v);
         std::sort(uniq.begin(), uniq.end());
         uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
         C.cap_slots = uniq;
-        H.c2f_cap.resize(srcs.size());
-        for (size_t k = 0; k < srcs.size(); ++k)
-            H.c2f_cap[k] = (uint32_t)(std::lower_bound(uniq.begin(), uniq.end(), srcs[k]) - uniq.begin());
+        H.c2f_cap.assign(src.size(), kNoCap);
+        for (size_t k = 0; k < src.size(); ++k)
+            if (src[k] >= 0)
+                H.c2f_cap[k] = (uint32_t)(std::lower_bound(uniq.begin(), uniq.end(), (uint32_t)src[k]) - uniq.begin());
     }
     for (int l = 0; l + 1 < NL; ++l) {
         HostLevel& H = S->h[l];

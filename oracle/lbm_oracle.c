@@ -15,7 +15,9 @@
  *  - Coarse step advance(l): f2c fill, capture pre-collision state of c2f
  *    sources (from the previous buffer), collide-stream, capture again, then
  *    two sub-steps of level l+1 each preceded by a c2f fill at alpha = 0, 1/2.
- *  - c2f: f_fine* = feq(rho,u) + (1-1/tau_f) (tau_f/(2 tau_c)) fneq_c.
+ *  - c2f: f_fine* = feq(rho_g,u_g) + (1-1/tau_f) (tau_f/(2 tau_c)) fneq_c, with
+ *    rho_g, u_g linearly interpolated from the coarse cell and its coarse
+ *    fluid neighbours.
  *  - f2c: average rho, j, fneq of the 8 children's pre-collision states;
  *    f_coarse* = feq + (1-1/tau_c) (2 tau_c/tau_f) avg fneq.
  */
@@ -75,7 +77,7 @@ struct or_lbm {
     or_lbm_params p;
     int L;              /* number of levels */
     level_t lv[24];
-    double c2f_mult, f2c_mult; /* 1.0; tests plant mistakes through these */
+    double c2f_mult, f2c_mult, slope_mult; /* 1.0; tests plant mistakes through these */
 };
 
 static inline int64_t pidx(const level_t* v, int x, int y, int z) {
@@ -135,6 +137,22 @@ static int source(const or_lbm* s, const level_t* v, int x, int y, int z, int i,
         *sx = p[0]; *sy = p[1]; *sz = p[2];
     }
     return 1;
+}
+
+/* Coarse fluid neighbour of coarse position cp one cell along axis a in
+ * direction d (+1/-1), across periodic faces; -1 when outside the domain or
+ * not a fluid leaf. */
+static int64_t coarse_nb(const or_lbm* s, const level_t* c, int64_t cp, int a, int d) {
+    int q[3];
+    pxyz(c, cp, &q[0], &q[1], &q[2]);
+    const int e[3] = {c->X, c->Y, c->Z};
+    q[a] += d;
+    if (q[a] < 0 || q[a] >= e[a]) {
+        if (s->p.bc[2 * a + (d > 0)] != OR_BC_PERIODIC) return -1;
+        q[a] = (q[a] + e[a]) % e[a];
+    }
+    int64_t np = pidx(c, q[0], q[1], q[2]);
+    return c->type[np] == T_FLUID ? np : -1;
 }
 
 static void gather(const or_lbm* s, const level_t* v, int64_t pos, const double* buf, double* out) {
@@ -202,6 +220,7 @@ or_lbm* or_lbm_create(const or_lbm_params* p, const int32_t* x, const int32_t* y
     s->L = p->max_level + 1;
     s->c2f_mult = 1.0;
     s->f2c_mult = 1.0;
+    s->slope_mult = 1.0;
     for (int l = 0; l < s->L; ++l) {
         level_t* v = &s->lv[l];
         v->X = p->nx0 << l; v->Y = p->ny0 << l; v->Z = p->nz0 << l;
@@ -237,6 +256,20 @@ or_lbm* or_lbm_create(const or_lbm_params* p, const int32_t* x, const int32_t* y
                     *err = 4; or_lbm_destroy(s); return NULL;
                 }
             }
+        }
+    }
+    /* the coarse fluid neighbours of every c2f source are captured too: the
+     * fill interpolates the coarse density and velocity linearly in space */
+    for (int l = 1; l < s->L; ++l) {
+        level_t* v = &s->lv[l];
+        level_t* c = &s->lv[l - 1];
+        for (size_t q = 0; q < v->N; ++q) {
+            if (v->type[q] != T_C2F) continue;
+            for (int a = 0; a < 3; ++a)
+                for (int d = -1; d <= 1; d += 2) {
+                    int64_t np = coarse_nb(s, c, v->c2f_src[q], a, d);
+                    if (np >= 0) c->is_cap[np] = 1;
+                }
         }
     }
     for (int l = 0; l < s->L; ++l) {
@@ -327,6 +360,18 @@ static void capture(or_lbm* s, int l, int slot) {
     }
 }
 
+static void cap_moments(const level_t* c, int64_t cp, double alpha, double* f, double* m) {
+    for (int i = 0; i < Q; ++i)
+        f[i] = (1.0 - alpha) * c->cap[0][(size_t)i * c->N + cp] + alpha * c->cap[1][(size_t)i * c->N + cp];
+    moments(f, &m[0], &m[1], &m[2], &m[3]);
+}
+
+/* c2f: density and velocity interpolated linearly from the covering coarse
+ * cell P to the fine cell centre (offset +-1/4 coarse cell per axis), with
+ * per-axis slopes from P's coarse fluid neighbours (central difference, or
+ * one-sided where only one side is a fluid leaf, or none); the
+ * non-equilibrium part is P's, rescaled:
+ *   f* = feq(rho_g, u_g) + (1 - 1/tau_f)(tau_f / (2 tau_c)) (f_P - feq(rho_P, u_P)) */
 static void fill_c2f(or_lbm* s, int l, double alpha) {
     level_t* v = &s->lv[l];
     level_t* c = &s->lv[l - 1];
@@ -336,13 +381,27 @@ static void fill_c2f(or_lbm* s, int l, double alpha) {
     for (int64_t k = 0; k < (int64_t)v->nc2f; ++k) {
         int64_t g = v->c2f[k];
         int64_t cp = v->c2f_src[g];
-        double fp[Q], r, ux, uy, uz;
-        for (int i = 0; i < Q; ++i)
-            fp[i] = (1.0 - alpha) * c->cap[0][(size_t)i * c->N + cp] + alpha * c->cap[1][(size_t)i * c->N + cp];
-        moments(fp, &r, &ux, &uy, &uz);
+        int gc[3];
+        pxyz(v, g, &gc[0], &gc[1], &gc[2]);
+        double fp[Q], mp[4], fn[Q], mm[4], mq[4], phi[4];
+        cap_moments(c, cp, alpha, fp, mp);
+        for (int j = 0; j < 4; ++j) phi[j] = mp[j];
+        for (int a = 0; a < 3; ++a) {
+            const int64_t nm = coarse_nb(s, c, cp, a, -1), nq = coarse_nb(s, c, cp, a, +1);
+            if (nm >= 0) cap_moments(c, nm, alpha, fn, mm);
+            if (nq >= 0) cap_moments(c, nq, alpha, fn, mq);
+            const double dl = ((gc[a] & 1) ? 0.25 : -0.25) * s->slope_mult;
+            for (int j = 0; j < 4; ++j) {
+                double gr = 0.0;
+                if (nm >= 0 && nq >= 0) gr = 0.5 * (mq[j] - mm[j]);
+                else if (nq >= 0) gr = mq[j] - mp[j];
+                else if (nm >= 0) gr = mp[j] - mm[j];
+                phi[j] += dl * gr;
+            }
+        }
         for (int i = 0; i < Q; ++i) {
-            double e = feq(r, ux, uy, uz, i);
-            dst[(size_t)i * v->N + g] = e + fac * (fp[i] - e);
+            const double e = feq(phi[0], phi[1], phi[2], phi[3], i);
+            dst[(size_t)i * v->N + g] = e + fac * (fp[i] - feq(mp[0], mp[1], mp[2], mp[3], i));
         }
     }
 }
@@ -464,9 +523,10 @@ void or_lbm_force(const or_lbm* s, double F[3]) {
     }
 }
 
-void or_lbm_plant_coupling_error(or_lbm* s, double c2f_mult, double f2c_mult) {
+void or_lbm_plant_coupling_error(or_lbm* s, double c2f_mult, double f2c_mult, double slope_mult) {
     s->c2f_mult = c2f_mult;
     s->f2c_mult = f2c_mult;
+    s->slope_mult = slope_mult;
 }
 
 /* ---- regrid support ------------------------------------------------------

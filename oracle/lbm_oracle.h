@@ -68,7 +68,7 @@ or_lbm* or_lbm_regrid(const or_lbm* o, const int32_t* x, const int32_t* y, const
 
 /* Tests only: multiply the c2f / f2c f_neq rescale factors (1.0 = the
  * scheme) so that a test can plant a plausible mistake and see it fail. */
-void or_lbm_plant_coupling_error(or_lbm* s, double c2f_mult, double f2c_mult);
+void or_lbm_plant_coupling_error(or_lbm* s, double c2f_mult, double f2c_mult, double slope_mult);
 
 /* D3Q19 constants as used by the oracle (for tests) */
 void or_lbm_lattice(int* c /*19*3*/, double* w /*19*/, int* opp /*19*/);

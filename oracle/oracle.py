@@ -112,7 +112,7 @@ def lib():
         L.or_lbm_totals.argtypes = [vp, vp, vp]
         L.or_lbm_force.argtypes = [vp, vp]
         L.or_lbm_lattice.argtypes = [vp, vp, vp]
-        L.or_lbm_plant_coupling_error.argtypes = [vp, C.c_double, C.c_double]
+        L.or_lbm_plant_coupling_error.argtypes = [vp, C.c_double, C.c_double, C.c_double]
         L.or_lbm_vorticity.argtypes = [vp, vp, vp, vp, vp, sz, vp]
         L.or_lbm_regrid.restype = vp
         L.or_lbm_regrid.argtypes = [vp, vp, vp, vp, vp, vp, sz, C.POINTER(C.c_int)]
@@ -383,8 +383,8 @@ class Lbm:
     def step(self, n=1):
         lib().or_lbm_step(self.h, n)
 
-    def plant_coupling_error(self, c2f_mult, f2c_mult):
-        lib().or_lbm_plant_coupling_error(self.h, c2f_mult, f2c_mult)
+    def plant_coupling_error(self, c2f_mult, f2c_mult, slopes=1.0):
+        lib().or_lbm_plant_coupling_error(self.h, c2f_mult, f2c_mult, slopes)
 
     def get_f(self):
         out = np.empty((len(self.fx), 19), np.float64)

@@ -128,5 +128,32 @@ def test_f16c_codec_matches_format(fl, oracle):
     g.set_f(w * (1.0 + x))
     assert np.allclose(g.get_f() / w - 1.0, x, rtol=1e-4, atol=0)
     xr = rng.choice([-1.0, 1.0], (n, 19)) * 2.0 ** rng.uniform(-8, -1, (n, 19))
-    g.set_f(w * (1.0 + xr))
-    assert np.allclose(g.get_f() / w - 1.0, F.decode(F.encode_rne(xr)), rtol=1e-4, atol=0)
+    f = w * (1.0 + xr)
+    g.set_f(f)
+    # the device encodes x = (float(f) - float(w)) * (1/w) in FP32
+    w32 = w.astype(np.float32)
+    x32 = ((f.astype(np.float32) - w32) * (1.0 / w).astype(np.float32)).astype(np.float64)
+    assert np.allclose(g.get_f() / w - 1.0, F.decode(F.encode_rne(x32)), rtol=1e-4, atol=0)
+
+
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-11), ("f32", 1e-4)])
+def test_couette_exact_across_refined_band(fl, oracle, storage, tol):
+    # linear coarse->fine interpolation carries the linear Couette profile
+    # through a refined band unchanged (the oracle holds it to 1e-11)
+    from fluidloom_amd.solver import LbmSolver
+    from tests.amr_util import box_region, refine_mesh, uniform_cells
+    from tests.lbm_cases import PER, WALL, ZERO
+
+    U, N = 0.02, 8
+    bcu = list(ZERO)
+    bcu[5] = (U, 0.0, 0.0)
+    base = uniform_cells(oracle, (4, 4, N))
+    c = refine_mesh(oracle, base, (4, 4, N), [(1, box_region((0, 0, N // 2 - 1), (4, 4, N // 2 + 1)))], 1)
+    g = LbmSolver(4, 4, N, 1, 1.0, [PER, PER, PER, PER, WALL, WALL], bcu, c.x, c.y, c.z, c.level, c.state,
+                  storage=storage)
+    g.step(1500)
+    _, _, z, lv = (np.asarray(a) for a in g.fluid[0])
+    _, u = g.moments()
+    exact = U * ((z + 0.5) / 2.0 ** lv) / N
+    assert np.abs(u[:, 0] - exact).max() / U < tol
+    g.close()

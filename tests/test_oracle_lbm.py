@@ -98,12 +98,28 @@ def test_couette_profile_exact_on_uniform_grid(oracle):
     assert _couette(oracle, 8, False, 1500) < 1e-9
 
 
-def test_couette_profile_first_order_across_interfaces(oracle):
-    # the coarse->fine fill copies the parent's moments (zeroth order in
-    # space), so the interface error is O(dx): halving dx halves it
-    e8 = _couette(oracle, 8, True, 1500)
-    e16 = _couette(oracle, 16, True, 12000)
-    assert e8 < 0.03 and e16 < 0.6 * e8
+def test_couette_profile_exact_across_interfaces(oracle):
+    # the coarse->fine fill interpolates density and velocity linearly from
+    # the coarse neighbours and rescales the (uniform) shear stress, so the
+    # linear Couette profile crosses a refined band unchanged
+    assert _couette(oracle, 8, True, 1500) < 1e-11
+    assert _couette(oracle, 16, True, 12000) < 1e-11
+
+
+def test_couette_detects_a_zeroth_order_fill(oracle):
+    # the same band with the fill's slopes switched off (planted in the
+    # oracle) gives the O(dx) interface error of a parent copy
+    U, N = 0.02, 8
+    bcu = list(ZERO)
+    bcu[5] = (U, 0.0, 0.0)
+    base = uniform_cells(oracle, (4, 4, N))
+    c = refine_mesh(oracle, base, (4, 4, N), [(1, box_region((0, 0, N // 2 - 1), (4, 4, N // 2 + 1)))], 1)
+    s = oracle.Lbm(4, 4, N, 1, 1.0, [PER, PER, PER, PER, WALL, WALL], bcu, c.x, c.y, c.z, c.level, c.state)
+    s.plant_coupling_error(1.0, 1.0, slopes=0.0)
+    s.step(1500)
+    _, u = s.moments()
+    err = np.abs(u[:, 0] - U * ((s.fz + 0.5) / (2.0 ** s.flv)) / N).max() / U
+    assert err > 1e-3
 
 
 def test_uniform_flow_through_coarse_fine_interfaces(oracle):
