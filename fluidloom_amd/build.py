@@ -29,6 +29,10 @@ COMMON = [
     f"--offload-arch={ARCH}",
     # keep packed FP32 instructions out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # fuse multiply-adds only within one source expression, so that every
+    # launch variant of a kernel rounds the same way (cross-statement fusion
+    # depends on the code shape around it)
+    "-ffp-contract=on",
     "-Wall",
     "-Wno-unused-function",
     "-Wno-unused-variable",

@@ -25,15 +25,18 @@ namespace {
 // Plain streaming copy, 4 x 16 B per lane with all loads issued before the
 // stores: the bandwidth reference (and HBM-counter calibration) for the
 // other kernels.
-__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+// Streaming loads and stores (non-temporal): the fastest plain copy measured
+// on this part (6.28 TB/s for 2 GiB against 5.95 TB/s with cached accesses).
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy16(const v4u* __restrict__ src, v4u* __restrict__ dst, size_t n) {
     const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
-    uint4 v[4];
+    v4u v[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-        if (base + 256 * k < n) v[k] = src[base + 256 * k];
+        if (base + 256 * k < n) v[k] = __builtin_nontemporal_load(src + base + 256 * k);
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-        if (base + 256 * k < n) dst[base + 256 * k] = v[k];
+        if (base + 256 * k < n) __builtin_nontemporal_store(v[k], dst + base + 256 * k);
 }
 }  // namespace
 
@@ -114,8 +117,8 @@ fl_status fl_copy_kernel(void* d_dst, const void* d_src, size_t bytes, fl_stream
     if (n == 0) return FL_OK;
     const size_t g = (n + 1023) / 1024;
     FL_REQUIRE(g < (1ull << 31), "fl_copy_kernel: copy too large for one launch");
-    hipLaunchKernelGGL(k_copy16, dim3((unsigned)g), dim3(256), 0, FL_STREAM(s), static_cast<const uint4*>(d_src),
-                       static_cast<uint4*>(d_dst), n);
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)g), dim3(256), 0, FL_STREAM(s), static_cast<const v4u*>(d_src),
+                       static_cast<v4u*>(d_dst), n);
     FL_HIP(hipGetLastError());
     return FL_OK;
 }

@@ -8,8 +8,8 @@
 // tests/unit/parsing/test_parsing.cpp:23-29), then the twelve edge
 // directions in opposite pairs.  Weights 1/3, 1/18, 1/36; cs^2 = 1/3.
 //
-// Storage: populations are SoA f[q][slot] with slot = brick * 64 + local,
-// local = lx + 4 ly + 16 lz inside a 4x4x4 brick.  Three storage types:
+// Storage: populations are SoA f[q][slot] (pidx) with slot = brick * 64 +
+// local, local = lx + 4 ly + 16 lz inside a 4x4x4 brick.  Three storage types:
 // float, double, and _Float16 holding f_q - w_q.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -37,6 +37,9 @@ constexpr bool pairs_adjacent() {
     return opp(0) == 0;
 }
 static_assert(pairs_adjacent(), "lattice declaration must list opposite velocities in adjacent pairs");
+
+// Element index of population i of a slot in an array of planes S apart.
+__host__ __device__ constexpr size_t pidx(int i, size_t S, size_t slot) { return (size_t)i * S + slot; }
 
 // Storage codecs.
 template <typename T>
@@ -160,19 +163,28 @@ __device__ inline R feq(int i, R rho, R ux, R uy, R uz, R usq) {
 }
 
 template <typename R>
-__device__ inline void moments(const R* f, R& rho, R& ux, R& uy, R& uz) {
-    R r = 0, jx = 0, jy = 0, jz = 0;
+__device__ inline void density_momentum(const R* f, R& rho, R& jx, R& jy, R& jz) {
+    R r = 0, ax = 0, ay = 0, az = 0;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         r += f[i];
-        if (cx(i)) jx += cx(i) > 0 ? f[i] : -f[i];
-        if (cy(i)) jy += cy(i) > 0 ? f[i] : -f[i];
-        if (cz(i)) jz += cz(i) > 0 ? f[i] : -f[i];
+        if (cx(i)) ax += cx(i) > 0 ? f[i] : -f[i];
+        if (cy(i)) ay += cy(i) > 0 ? f[i] : -f[i];
+        if (cz(i)) az += cz(i) > 0 ? f[i] : -f[i];
     }
     rho = r;
-    ux = jx / r;
-    uy = jy / r;
-    uz = jz / r;
+    jx = ax;
+    jy = ay;
+    jz = az;
+}
+
+template <typename R>
+__device__ inline void moments(const R* f, R& rho, R& ux, R& uy, R& uz) {
+    R jx, jy, jz;
+    density_momentum(f, rho, jx, jy, jz);
+    ux = jx / rho;
+    uy = jy / rho;
+    uz = jz / rho;
 }
 
 // BGK: f* = f - (f - feq) / tau (in place)
@@ -194,6 +206,8 @@ constexpr uint64_t kActiveBit = 1ull << 63;
 // c2f capture indices per ghost: the covering coarse cell, then its coarse
 // fluid neighbours along -x, +x, -y, +y, -z, +z (kNoCap when absent)
 constexpr int kC2FCaps = 7;
+// planes of a capture buffer: the Q populations, then density and momentum
+constexpr int kCapPlanes = Q + 4;
 constexpr uint32_t kNoCap = 0xFFFFFFFFu;
 enum : unsigned { kLinkRegular = 0, kLinkSolid = 1, kLinkWall = 2, kLinkOutflow = 3 };
 
@@ -253,8 +267,7 @@ __device__ inline int64_t slot_of(const LevelDev& L, int b, int px, int py, int 
 template <typename T, typename R>
 __device__ inline R special_link(const LevelDev& L, const T* __restrict__ A, int b, size_t slot, int i, unsigned code,
                                  int px, int py, int pz) {
-    const size_t S = L.S;
-    if (code == kLinkSolid) return Store<T>::load(A, (size_t)opp(i) * S + slot, opp(i));
+    if (code == kLinkSolid) return Store<T>::load(A, pidx(opp(i), L.S, slot), opp(i));
     const InstDev& I = L.inst[L.binst[b]];
     const int32_t* c3 = L.bcoord + 3 * b;
     if (code == kLinkWall) {
@@ -267,7 +280,7 @@ __device__ inline R special_link(const LevelDev& L, const T* __restrict__ A, int
                 uw = (R)((float)cx(i) * I.bc_u[face][0] + (float)cy(i) * I.bc_u[face][1] +
                          (float)cz(i) * I.bc_u[face][2]);
         }
-        return Store<T>::load(A, (size_t)opp(i) * S + slot, opp(i)) + (R)6 * (R)wq(i) * (R)I.rho0 * uw;
+        return Store<T>::load(A, pidx(opp(i), L.S, slot), opp(i)) + (R)6 * (R)wq(i) * (R)I.rho0 * uw;
     }
     const int ex = I.ext0[0] << L.level, ey = I.ext0[1] << L.level, ez = I.ext0[2] << L.level;
     const int gx = c3[0] * 4 + px, gy = c3[1] * 4 + py, gz = c3[2] * 4 + pz;
@@ -275,7 +288,7 @@ __device__ inline R special_link(const LevelDev& L, const T* __restrict__ A, int
     const int qy = gy < 0 ? 0 : (gy >= ey ? ey - 1 : gy);
     const int qz = gz < 0 ? 0 : (gz >= ez ? ez - 1 : gz);
     const int64_t s2 = slot_of(L, b, qx - c3[0] * 4, qy - c3[1] * 4, qz - c3[2] * 4);
-    return Store<T>::load(A, (size_t)i * S + (size_t)s2, i);
+    return Store<T>::load(A, pidx(i, L.S, (size_t)s2), i);
 }
 
 // Generic pull of the 19 incoming populations of one slot (list kernels).
@@ -290,7 +303,7 @@ __device__ inline void gather_slot(const LevelDev& L, const T* __restrict__ A, i
         const unsigned code = (unsigned)(w >> (2 * i)) & 3u;
         const int px = lx - cx(i), py = ly - cy(i), pz = lz - cz(i);
         if (code == kLinkRegular)
-            f[i] = Store<T>::load(A, (size_t)i * L.S + (size_t)slot_of(L, b, px, py, pz), i);
+            f[i] = Store<T>::load(A, pidx(i, L.S, (size_t)slot_of(L, b, px, py, pz)), i);
         else
             f[i] = special_link<T, R>(L, A, b, (size_t)slot, i, code, px, py, pz);
     }

@@ -71,9 +71,8 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
     const bool active = simple || (w & kActiveBit);
     const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
     const size_t S = L.S;
-    R f[Q];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
+    // pulled source of direction i: slot inside plane i (A + i*S)
+    auto source = [&](int i) -> unsigned {
         const int px = lx - cx(i), py = ly - cy(i), pz = lz - cz(i);
         const bool ex = (unsigned)px > 3u, ey = (unsigned)py > 3u, ez = (unsigned)pz > 3u;
         const int kx = -cx(i), ky = -3 * cy(i), kz = -9 * cz(i);
@@ -81,14 +80,27 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
         const int n001 = NB[13 + kz], n101 = NB[13 + kx + kz], n011 = NB[13 + ky + kz], n111 = NB[13 + kx + ky + kz];
         const int nb = ez ? (ey ? (ex ? n111 : n011) : (ex ? n101 : n001))
                           : (ey ? (ex ? n110 : n010) : (ex ? n100 : n000));
-        const unsigned src = (unsigned)nb * 64u + (unsigned)((px & 3) + 4 * (py & 3) + 16 * (pz & 3));
-        f[i] = 0;
-        if (active) {
-            const unsigned code = (unsigned)(w >> (2 * i)) & 3u;
-            if (code == kLinkRegular)
-                f[i] = Store<T>::load(A + (size_t)i * S, src, i);
-            else
-                f[i] = special_link<T, R>(L, A, b, slot, i, code, px, py, pz);
+        return (unsigned)nb * 64u + (unsigned)((px & 3) + 4 * (py & 3) + 16 * (pz & 3));
+    };
+    R f[Q];
+    if (std::is_same<T, _Float16>::value && simple) {
+        // FP16 storage, every cell fluid, every link regular: 19 independent
+        // loads back to back.  The other storage types keep the interleaved
+        // form below: it measured faster for FP32, and the FP16C decode in
+        // this form needs more registers than one wave slot holds.
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f[i] = Store<T>::load(A + (size_t)i * S, source(i), i);
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            f[i] = 0;
+            if (active) {
+                const unsigned code = (unsigned)(w >> (2 * i)) & 3u;
+                if (code == kLinkRegular)
+                    f[i] = Store<T>::load(A + (size_t)i * S, source(i), i);
+                else
+                    f[i] = special_link<T, R>(L, A, b, slot, i, code, lx - cx(i), ly - cy(i), lz - cz(i));
+            }
         }
     }
     if (!active) return;
@@ -102,10 +114,13 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
         const uint32_t seed = mix32(gx * 0x9E3779B1u ^ mix32(gy * 0x85EBCA77u ^ mix32(gz * 0xC2B2AE3Du ^
                                     ((uint32_t)L.level << 24) ^ (*L.tick * 0x27D4EB2Fu))));
 #pragma unroll
-        for (int i = 0; i < Q; ++i) Store<T>::store_sr(B + (size_t)i * S, slot, i, f[i], mix32(seed + 0x9E3779B9u * (uint32_t)(i + 1)));
+        for (int i = 0; i < Q; ++i)
+            Store<T>::store_sr(B + (size_t)i * S, slot, i, f[i],
+                               mix32(seed + 0x9E3779B9u * (uint32_t)(i + 1)));
     } else {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) Store<T>::store_nt(B + (size_t)i * S, slot, i, f[i]);
+        for (int i = 0; i < Q; ++i)
+            Store<T>::store_nt(B + (size_t)i * S, slot, i, f[i]);
     }
 }
 
@@ -138,10 +153,13 @@ __global__ __launch_bounds__(kWG) void k_capture(LevelDev L, const T* __restrict
                                                  typename Store<T>::R* __restrict__ cap) {
     using R = typename Store<T>::R;
     for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < ncap; k += gridDim.x * kWG) {
-        R f[Q];
+        R f[Q], m[4];
         gather_slot<T, R>(L, A, cap_slots[k], f);
+        density_momentum(f, m[0], m[1], m[2], m[3]);
 #pragma unroll
         for (int i = 0; i < Q; ++i) cap[(size_t)i * ncap + k] = f[i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) cap[(size_t)(Q + j) * ncap + k] = m[j];
     }
 }
 
@@ -155,58 +173,93 @@ __device__ inline double f2c_factor(double tau0, int l) {
     return (1.0 - 1.0 / tc) * (2.0 * tc / tf);
 }
 
-// Time-interpolated captured state of one coarse cell and its moments
-// (rho, ux, uy, uz).
-template <typename R>
-__device__ inline void cap_state(const R* __restrict__ cap0, const R* __restrict__ cap1, uint32_t ncap, uint32_t c,
-                                 R alpha, R* f, R* m) {
-#pragma unroll
-    for (int i = 0; i < Q; ++i) f[i] = ((R)1 - alpha) * cap0[(size_t)i * ncap + c] + alpha * cap1[(size_t)i * ncap + c];
-    moments(f, m[0], m[1], m[2], m[3]);
-}
-
+// Coarse -> fine, eight lanes per group of ghosts sharing a covering coarse
+// leaf P (lane q = the child at fine-coordinate parity x + 2y + 4z).  The
+// group loads P's time-interpolated populations and its neighbours'
+// time-interpolated density and momentum once, three values per lane, and
+// shares them with shuffles.  Each ghost then takes P's moments plus a
+// linear correction along each axis to its own centre (offset 1/4 of a
+// coarse cell; central slope when both coarse neighbours along the axis are
+// fluid leaves, one-sided when one is, none otherwise) and P's rescaled
+// non-equilibrium part.
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_fill_c2f(LevelDev Lf, const uint32_t* __restrict__ ghost,
-                                                  const uint32_t* __restrict__ capidx, uint32_t n,
+__global__ __launch_bounds__(kWG) void k_fill_c2f(LevelDev Lf, const uint32_t* __restrict__ gchild,
+                                                  const uint32_t* __restrict__ gcap, uint32_t ngroups,
                                                   const typename Store<T>::R* __restrict__ cap0,
                                                   const typename Store<T>::R* __restrict__ cap1, uint32_t ncap,
                                                   typename Store<T>::R alpha, T* __restrict__ A) {
     using R = typename Store<T>::R;
-    for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < n; k += gridDim.x * kWG) {
-        const size_t g = ghost[k];
-        const InstDev& I = Lf.inst[Lf.binst[g >> 6]];
-        if (!I.live) continue;
-        const R fac = (R)c2f_factor(I.tau0, Lf.level);
-        // capture indices: covering coarse cell P, then its coarse fluid
-        // neighbours -x, +x, -y, +y, -z, +z (kNoCap where there is none)
-        const uint32_t* ci = capidx + (size_t)kC2FCaps * k;
-        R f[Q], mp[4];
-        cap_state(cap0, cap1, ncap, ci[0], alpha, f, mp);
-        R phi[4] = {mp[0], mp[1], mp[2], mp[3]};
-        const int loc = (int)(g & 63);
-        const int par[3] = {loc & 1, (loc >> 2) & 1, (loc >> 4) & 1};  // fine-cell parity per axis
-        for (int a = 0; a < 3; ++a) {
-            const uint32_t nm = ci[1 + 2 * a], nq = ci[2 + 2 * a];
-            R fn[Q], mm[4] = {0, 0, 0, 0}, mq[4] = {0, 0, 0, 0};
-            if (nm != kNoCap) cap_state(cap0, cap1, ncap, nm, alpha, fn, mm);
-            if (nq != kNoCap) cap_state(cap0, cap1, ncap, nq, alpha, fn, mq);
-            const R dl = par[a] ? (R)0.25 : (R)-0.25;
+    const uint32_t t = blockIdx.x * kWG + threadIdx.x;
+    const uint32_t k = t >> 3;
+    const int q = (int)(t & 7u);
+    // every lane of a group shares k, so a group is wholly in or out
+    const bool on = k < ngroups;
+    uint32_t ci[kC2FCaps];
+    uint32_t g = kNoCap;
+    R pv[3] = {0, 0, 0}, mv[3] = {0, 0, 0};
+    if (on) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                R gr = 0;
-                if (nm != kNoCap && nq != kNoCap) gr = (R)0.5 * (mq[j] - mm[j]);
-                else if (nq != kNoCap) gr = mq[j] - mp[j];
-                else if (nm != kNoCap) gr = mp[j] - mm[j];
-                phi[j] += dl * gr;
-            }
-        }
-        const R usq = phi[1] * phi[1] + phi[2] * phi[2] + phi[3] * phi[3];
-        const R usp = mp[1] * mp[1] + mp[2] * mp[2] + mp[3] * mp[3];
+        for (int j = 0; j < kC2FCaps; ++j) ci[j] = gcap[(size_t)kC2FCaps * k + j];
+        g = gchild[(size_t)8 * k + q];
 #pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            const R e = feq(i, phi[0], phi[1], phi[2], phi[3], usq);
-            Store<T>::store(A, (size_t)i * Lf.S + g, i, e + fac * (f[i] - feq(i, mp[0], mp[1], mp[2], mp[3], usp)));
+        for (int r = 0; r < 3; ++r) {
+            const int i = q + 8 * r;  // population i of P
+            if (i < Q) pv[r] = ((R)1 - alpha) * cap0[(size_t)i * ncap + ci[0]] + alpha * cap1[(size_t)i * ncap + ci[0]];
+            const int j = q + 8 * r;  // moment j & 3 of neighbour j >> 2 (j < 24)
+            const uint32_t c = ci[1 + (j >> 2)];
+            const size_t m = (size_t)(Q + (j & 3)) * ncap + c;
+            if (c != kNoCap) mv[r] = ((R)1 - alpha) * cap0[m] + alpha * cap1[m];
         }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kC2FCaps; ++j) ci[j] = kNoCap;
+    }
+    R f[Q], nm[24];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = __shfl(pv[i >> 3], i & 7, 8);
+#pragma unroll
+    for (int j = 0; j < 24; ++j) nm[j] = __shfl(mv[j >> 3], j & 7, 8);
+    if (g == kNoCap) return;
+    const InstDev& I = Lf.inst[Lf.binst[g >> 6]];
+    if (!I.live) return;
+    const R fac = (R)c2f_factor(I.tau0, Lf.level);
+    R mp[4];
+    moments(f, mp[0], mp[1], mp[2], mp[3]);
+    R phi[4] = {mp[0], mp[1], mp[2], mp[3]};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const bool hm = ci[1 + 2 * a] != kNoCap, hq = ci[2 + 2 * a] != kNoCap;
+        R mm[4] = {0, 0, 0, 0}, mq[4] = {0, 0, 0, 0};
+        if (hm) {
+            const R* v = nm + 4 * (2 * a);
+            mm[0] = v[0];
+            mm[1] = v[1] / v[0];
+            mm[2] = v[2] / v[0];
+            mm[3] = v[3] / v[0];
+        }
+        if (hq) {
+            const R* v = nm + 4 * (2 * a + 1);
+            mq[0] = v[0];
+            mq[1] = v[1] / v[0];
+            mq[2] = v[2] / v[0];
+            mq[3] = v[3] / v[0];
+        }
+        const R dl = ((q >> a) & 1) ? (R)0.25 : (R)-0.25;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            R gr = 0;
+            if (hm && hq) gr = (R)0.5 * (mq[j] - mm[j]);
+            else if (hq) gr = mq[j] - mp[j];
+            else if (hm) gr = mp[j] - mm[j];
+            phi[j] += dl * gr;
+        }
+    }
+    const R usq = phi[1] * phi[1] + phi[2] * phi[2] + phi[3] * phi[3];
+    const R usp = mp[1] * mp[1] + mp[2] * mp[2] + mp[3] * mp[3];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const R e = feq(i, phi[0], phi[1], phi[2], phi[3], usq);
+        Store<T>::store(A, pidx(i, Lf.S, g), i, e + fac * (f[i] - feq(i, mp[0], mp[1], mp[2], mp[3], usp)));
     }
 }
 
@@ -239,23 +292,53 @@ __device__ inline void child_average(const LevelDev& Lf, const T* __restrict__ A
     for (int i = 0; i < Q; ++i) aneq[i] *= (R)0.125;
 }
 
+// Fine -> coarse: eight lanes per ghost, lane q pulling child q; the group
+// sums density, momentum and non-equilibrium parts with xor shuffles (a
+// fixed tree, so the result does not depend on the launch) and each lane
+// then stores up to three of the ghost's 19 populations.
 template <typename T>
 __global__ __launch_bounds__(kWG) void k_fill_f2c(LevelDev Lc, const uint32_t* __restrict__ ghost,
                                                   const uint32_t* __restrict__ child8, uint32_t n, LevelDev Lf,
                                                   const T* __restrict__ Af, T* __restrict__ Ac) {
     using R = typename Store<T>::R;
-    for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < n; k += gridDim.x * kWG) {
-        const size_t g = ghost[k];
-        const InstDev& I = Lc.inst[Lc.binst[g >> 6]];
-        if (!I.live) continue;
-        const R fac = (R)f2c_factor(I.tau0, Lc.level);
-        R r, ux, uy, uz, aneq[Q];
-        child_average<T>(Lf, Af, child8 + (size_t)8 * k, r, ux, uy, uz, aneq);
-        const R usq = ux * ux + uy * uy + uz * uz;
-#pragma unroll
-        for (int i = 0; i < Q; ++i)
-            Store<T>::store(Ac, (size_t)i * Lc.S + g, i, feq(i, r, ux, uy, uz, usq) + fac * aneq[i]);
+    const uint32_t t = blockIdx.x * kWG + threadIdx.x;
+    const uint32_t k = t >> 3;
+    const int q = (int)(t & 7u);
+    // every lane of a group shares k, so a group is wholly in or out
+    bool on = k < n;
+    size_t g = 0;
+    if (on) {
+        g = ghost[k];
+        on = Lc.inst[Lc.binst[g >> 6]].live != 0;
     }
+    R s[4 + Q];
+#pragma unroll
+    for (int j = 0; j < 4 + Q; ++j) s[j] = 0;
+    if (on) {
+        R f[Q];
+        gather_slot<T, R>(Lf, Af, child8[(size_t)8 * k + q], f);
+        R rho, vx, vy, vz;
+        moments(f, rho, vx, vy, vz);
+        const R usq = vx * vx + vy * vy + vz * vz;
+        s[0] = rho;
+        s[1] = rho * vx;
+        s[2] = rho * vy;
+        s[3] = rho * vz;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) s[4 + i] = f[i] - feq(i, rho, vx, vy, vz, usq);
+    }
+#pragma unroll
+    for (int m = 1; m < 8; m <<= 1)
+#pragma unroll
+        for (int j = 0; j < 4 + Q; ++j) s[j] += __shfl_xor(s[j], m, 8);
+    if (!on) return;
+    const R fac = (R)f2c_factor(Lc.inst[Lc.binst[g >> 6]].tau0, Lc.level);
+    const R r = s[0] * (R)0.125;
+    const R ux = (s[1] * (R)0.125) / r, uy = (s[2] * (R)0.125) / r, uz = (s[3] * (R)0.125) / r;
+    const R usq = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+        if ((i & 7) == q) Store<T>::store(Ac, pidx(i, Lc.S, g), i, feq(i, r, ux, uy, uz, usq) + fac * (s[4 + i] * (R)0.125));
 }
 
 template <typename T>
@@ -268,8 +351,8 @@ __global__ __launch_bounds__(kWG) void k_init(LevelDev L, T* __restrict__ A, T* 
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             R e = feq(i, rho, ux, uy, uz, usq);
-            Store<T>::store(A, (size_t)i * L.S + s, i, e);
-            Store<T>::store(B, (size_t)i * L.S + s, i, e);
+            Store<T>::store(A, pidx(i, L.S, s), i, e);
+            Store<T>::store(B, pidx(i, L.S, s), i, e);
         }
     }
 }
@@ -298,7 +381,7 @@ __global__ __launch_bounds__(kWG) void k_brick_sums(LevelDev L, const T* __restr
     if (act) {
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            const double f = (double)Store<T>::load(A, (size_t)i * L.S + s, i);
+            const double f = (double)Store<T>::load(A, pidx(i, L.S, s), i);
             r += f;
             jx += cx(i) * f;
             jy += cy(i) * f;
@@ -331,7 +414,7 @@ __global__ __launch_bounds__(kWG) void k_brick_du(LevelDev L, const T* __restric
         double r = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            const double f = (double)Store<T>::load(A, (size_t)i * L.S + s, i);
+            const double f = (double)Store<T>::load(A, pidx(i, L.S, s), i);
             r += f;
             jx += cx(i) * f;
             jy += cy(i) * f;
@@ -391,7 +474,7 @@ __global__ __launch_bounds__(kWG) void k_link_force(const uint32_t* __restrict__
 #pragma unroll
         for (int i = 1; i < Q; ++i)
             if (i == q) {
-                fo = (double)Store<T>::load(A, (size_t)opp(i) * S + s, opp(i));
+                fo = (double)Store<T>::load(A, pidx(opp(i), S, s), opp(i));
                 c0 = cx(i);
                 c1 = cy(i);
                 c2 = cz(i);
@@ -408,7 +491,7 @@ __global__ __launch_bounds__(kWG) void k_extract(const T* __restrict__ A, size_t
     for (size_t k = (size_t)blockIdx.x * kWG + threadIdx.x; k < n; k += (size_t)gridDim.x * kWG) {
         const int64_t s = slots[k];
 #pragma unroll
-        for (int i = 0; i < Q; ++i) out[k * Q + i] = s < 0 ? 0.0 : (double)Store<T>::load(A, (size_t)i * S + s, i);
+        for (int i = 0; i < Q; ++i) out[k * Q + i] = s < 0 ? 0.0 : (double)Store<T>::load(A, pidx(i, S, s), i);
     }
 }
 
@@ -422,8 +505,8 @@ __global__ __launch_bounds__(kWG) void k_inject(T* __restrict__ A, T* __restrict
         if (s < 0) continue;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            Store<T>::store(A, (size_t)i * S + s, i, (R)in[k * Q + i]);
-            Store<T>::store(B, (size_t)i * S + s, i, (R)in[k * Q + i]);
+            Store<T>::store(A, pidx(i, S, s), i, (R)in[k * Q + i]);
+            Store<T>::store(B, pidx(i, S, s), i, (R)in[k * Q + i]);
         }
     }
 }
@@ -437,7 +520,7 @@ __device__ inline bool leaf_velocity(const LevelDev& L, const T* __restrict__ A,
     double r = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-        double f = (double)Store<T>::load(A, (size_t)i * L.S + (size_t)s, i);
+        double f = (double)Store<T>::load(A, pidx(i, L.S, (size_t)s), i);
         r += f;
         jx += cx(i) * f;
         jy += cy(i) * f;
@@ -483,9 +566,9 @@ __global__ __launch_bounds__(kWG) void k_remap_copy(const T* __restrict__ Ao, si
     for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < n; k += gridDim.x * kWG) {
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            const T v = Ao[(size_t)i * So + src[k]];
-            A[(size_t)i * S + dst[k]] = v;
-            B[(size_t)i * S + dst[k]] = v;
+            const T v = Ao[pidx(i, So, src[k])];
+            A[pidx(i, S, dst[k])] = v;
+            B[pidx(i, S, dst[k])] = v;
         }
     }
 }
@@ -506,8 +589,8 @@ __global__ __launch_bounds__(kWG) void k_remap_split(LevelDev Ln, LevelDev Lc, c
         for (int i = 0; i < Q; ++i) {
             const R e = feq(i, rho, ux, uy, uz, usq);
             const R v = e + fac * (f[i] - e);
-            Store<T>::store(A, (size_t)i * Ln.S + dst[k], i, v);
-            Store<T>::store(B, (size_t)i * Ln.S + dst[k], i, v);
+            Store<T>::store(A, pidx(i, Ln.S, dst[k]), i, v);
+            Store<T>::store(B, pidx(i, Ln.S, dst[k]), i, v);
         }
     }
 }
@@ -525,8 +608,8 @@ __global__ __launch_bounds__(kWG) void k_remap_merge(LevelDev Ln, LevelDev Lf, c
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const R v = feq(i, r, ux, uy, uz, usq) + fac * aneq[i];
-            Store<T>::store(A, (size_t)i * Ln.S + dst[k], i, v);
-            Store<T>::store(B, (size_t)i * Ln.S + dst[k], i, v);
+            Store<T>::store(A, pidx(i, Ln.S, dst[k]), i, v);
+            Store<T>::store(B, pidx(i, Ln.S, dst[k]), i, v);
         }
     }
 }
@@ -539,8 +622,8 @@ __global__ __launch_bounds__(kWG) void k_remap_rest(const uint32_t* __restrict__
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const R v = (R)wq(i);
-            Store<T>::store(A, (size_t)i * S + dst[k], i, v);
-            Store<T>::store(B, (size_t)i * S + dst[k], i, v);
+            Store<T>::store(A, pidx(i, S, dst[k]), i, v);
+            Store<T>::store(B, pidx(i, S, dst[k]), i, v);
         }
     }
 }
@@ -548,24 +631,27 @@ __global__ __launch_bounds__(kWG) void k_remap_rest(const uint32_t* __restrict__
 // Ghost slots after a regrid: equilibrium of the covering coarse leaf (c2f)
 // or of the 8 children's average density and momentum (f2c), both buffers.
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_ghost_eq_c2f(const uint32_t* __restrict__ ghost, const uint32_t* __restrict__ capidx,
-                                                      const uint32_t* __restrict__ cap_slots, uint32_t n,
+__global__ __launch_bounds__(kWG) void k_ghost_eq_c2f(const uint32_t* __restrict__ gchild, const uint32_t* __restrict__ gcap,
+                                                      const uint32_t* __restrict__ cap_slots, uint32_t ngroups,
                                                       const T* __restrict__ Ac, size_t Sc, T* __restrict__ A,
                                                       T* __restrict__ B, size_t S) {
     using R = typename Store<T>::R;
-    for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < n; k += gridDim.x * kWG) {
-        const size_t cs = cap_slots[capidx[(size_t)kC2FCaps * k]];
+    const size_t n = (size_t)ngroups * 8;
+    for (size_t t = (size_t)blockIdx.x * kWG + threadIdx.x; t < n; t += (size_t)gridDim.x * kWG) {
+        const uint32_t g = gchild[t];
+        if (g == kNoCap) continue;
+        const size_t cs = cap_slots[gcap[(t >> 3) * kC2FCaps]];
         R f[Q];
 #pragma unroll
-        for (int i = 0; i < Q; ++i) f[i] = Store<T>::load(Ac, (size_t)i * Sc + cs, i);
+        for (int i = 0; i < Q; ++i) f[i] = Store<T>::load(Ac, pidx(i, Sc, cs), i);
         R rho, ux, uy, uz;
         moments(f, rho, ux, uy, uz);
         const R usq = ux * ux + uy * uy + uz * uz;
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const R e = feq(i, rho, ux, uy, uz, usq);
-            Store<T>::store(A, (size_t)i * S + ghost[k], i, e);
-            Store<T>::store(B, (size_t)i * S + ghost[k], i, e);
+            Store<T>::store(A, pidx(i, S, g), i, e);
+            Store<T>::store(B, pidx(i, S, g), i, e);
         }
     }
 }
@@ -581,7 +667,7 @@ __global__ __launch_bounds__(kWG) void k_ghost_eq_f2c(const uint32_t* __restrict
             const size_t cs = child8[(size_t)8 * k + q];
             R f[Q];
 #pragma unroll
-            for (int i = 0; i < Q; ++i) f[i] = Store<T>::load(Af, (size_t)i * Sf + cs, i);
+            for (int i = 0; i < Q; ++i) f[i] = Store<T>::load(Af, pidx(i, Sf, cs), i);
             R rho, ux, uy, uz;
             moments(f, rho, ux, uy, uz);
             sr += rho;
@@ -595,8 +681,8 @@ __global__ __launch_bounds__(kWG) void k_ghost_eq_f2c(const uint32_t* __restrict
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const R e = feq(i, r, ux, uy, uz, usq);
-            Store<T>::store(A, (size_t)i * S + ghost[k], i, e);
-            Store<T>::store(B, (size_t)i * S + ghost[k], i, e);
+            Store<T>::store(A, pidx(i, S, ghost[k]), i, e);
+            Store<T>::store(B, pidx(i, S, ghost[k]), i, e);
         }
     }
 }
@@ -684,11 +770,13 @@ hipError_t launch_capture(const LevelDev& L, const T* A, const uint32_t* slots, 
 }
 
 template <typename T>
-hipError_t launch_fill_c2f(const LevelDev& Lf, const uint32_t* ghost, const uint32_t* capidx, uint32_t n,
+hipError_t launch_fill_c2f(const LevelDev& Lf, const uint32_t* gchild, const uint32_t* gcap, uint32_t ngroups,
                            const void* cap0, const void* cap1, uint32_t ncap, double alpha, T* A, hipStream_t s) {
-    if (n == 0) return hipSuccess;
+    if (ngroups == 0) return hipSuccess;
     using R = typename Store<T>::R;
-    hipLaunchKernelGGL(k_fill_c2f<T>, dim3(lgrid(n)), dim3(kWG), 0, s, Lf, ghost, capidx, n,
+    // eight lanes per group, one pass (no grid stride: a group must stay whole)
+    const size_t blocks = ((size_t)ngroups * 8 + kWG - 1) / kWG;
+    hipLaunchKernelGGL(k_fill_c2f<T>, dim3((unsigned)blocks), dim3(kWG), 0, s, Lf, gchild, gcap, ngroups,
                        static_cast<const R*>(cap0), static_cast<const R*>(cap1), ncap, (R)alpha, A);
     return hipGetLastError();
 }
@@ -697,7 +785,9 @@ template <typename T>
 hipError_t launch_fill_f2c(const LevelDev& Lc, const uint32_t* ghost, const uint32_t* child8, uint32_t n,
                            const LevelDev& Lf, const T* Af, T* Ac, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_f2c<T>, dim3(lgrid(n)), dim3(kWG), 0, s, Lc, ghost, child8, n, Lf, Af, Ac);
+    // eight lanes per ghost, one pass (no grid stride: a group must stay whole)
+    const size_t blocks = ((size_t)n * 8 + kWG - 1) / kWG;
+    hipLaunchKernelGGL(k_fill_f2c<T>, dim3((unsigned)blocks), dim3(kWG), 0, s, Lc, ghost, child8, n, Lf, Af, Ac);
     return hipGetLastError();
 }
 
@@ -777,11 +867,11 @@ hipError_t launch_remap(const RemapArgs<T>& r, hipStream_t s) {
 }
 
 template <typename T>
-hipError_t launch_ghost_eq(const uint32_t* c2f_ghost, const uint32_t* c2f_cap, const uint32_t* cap_slots, uint32_t nc2f,
+hipError_t launch_ghost_eq(const uint32_t* c2f_gchild, const uint32_t* c2f_gcap, const uint32_t* cap_slots, uint32_t nc2f,
                            const T* Ac, size_t Sc, const uint32_t* f2c_ghost, const uint32_t* f2c_child, uint32_t nf2c,
                            const T* Af, size_t Sf, T* A, T* B, size_t S, hipStream_t s) {
     if (nc2f)
-        hipLaunchKernelGGL(k_ghost_eq_c2f<T>, dim3(lgrid(nc2f)), dim3(kWG), 0, s, c2f_ghost, c2f_cap, cap_slots, nc2f,
+        hipLaunchKernelGGL(k_ghost_eq_c2f<T>, dim3(lgrid((size_t)nc2f * 8)), dim3(kWG), 0, s, c2f_gchild, c2f_gcap, cap_slots, nc2f,
                            Ac, Sc, A, B, S);
     if (nf2c)
         hipLaunchKernelGGL(k_ghost_eq_f2c<T>, dim3(lgrid(nf2c)), dim3(kWG), 0, s, f2c_ghost, f2c_child, nf2c, Af, Sf,

@@ -21,7 +21,7 @@ hipError_t launch_tick(uint32_t* tick, hipStream_t s);
 template <typename T>
 hipError_t launch_capture(const LevelDev& L, const T* A, const uint32_t* slots, uint32_t ncap, void* cap, hipStream_t s);
 template <typename T>
-hipError_t launch_fill_c2f(const LevelDev& Lf, const uint32_t* ghost, const uint32_t* capidx, uint32_t n,
+hipError_t launch_fill_c2f(const LevelDev& Lf, const uint32_t* gchild, const uint32_t* gcap, uint32_t ngroups,
                            const void* cap0, const void* cap1, uint32_t ncap, double alpha, T* A, hipStream_t s);
 template <typename T>
 hipError_t launch_fill_f2c(const LevelDev& Lc, const uint32_t* ghost, const uint32_t* child8, uint32_t n,
@@ -75,7 +75,7 @@ struct RemapArgs {
 template <typename T>
 hipError_t launch_remap(const RemapArgs<T>& r, hipStream_t s);
 template <typename T>
-hipError_t launch_ghost_eq(const uint32_t* c2f_ghost, const uint32_t* c2f_cap, const uint32_t* cap_slots, uint32_t nc2f,
+hipError_t launch_ghost_eq(const uint32_t* c2f_gchild, const uint32_t* c2f_gcap, const uint32_t* cap_slots, uint32_t nc2f,
                            const T* Ac, size_t Sc, const uint32_t* f2c_ghost, const uint32_t* f2c_child, uint32_t nf2c,
                            const T* Af, size_t Sf, T* A, T* B, size_t S, hipStream_t s);
 

@@ -155,7 +155,10 @@ struct HostLevel {
     std::vector<uint32_t> bflags;
     std::vector<uint64_t> cflags;
     std::vector<uint8_t> stype, needed;
-    std::vector<uint32_t> c2f_ghost, c2f_cap, cap_slots, f2c_ghost, f2c_child, force_links;
+    std::vector<uint32_t> c2f_ghost, cap_slots, f2c_ghost, f2c_child, force_links;
+    // c2f ghosts grouped by covering coarse leaf: 8 child slots per group
+    // (kNoCap where that child is not a ghost) and 7 capture indices
+    std::vector<uint32_t> c2f_gchild, c2f_gcap;
     std::vector<uint32_t> abeg;  // [ninst+1] active-brick range of each instance
     std::vector<uint32_t> lbeg;  // [ninst+1] solid-link range of each instance
 
@@ -185,8 +188,8 @@ struct DevLevel {
     uint16_t* binst = nullptr;
     uint32_t* bflags = nullptr;
     uint64_t* cflags = nullptr;
-    uint32_t* c2f_ghost = nullptr;
-    uint32_t* c2f_cap = nullptr;
+    uint32_t* c2f_gchild = nullptr;
+    uint32_t* c2f_gcap = nullptr;
     uint32_t* cap_slots = nullptr;
     uint32_t* f2c_ghost = nullptr;
     uint32_t* f2c_child = nullptr;
@@ -633,25 +636,53 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
             if (H.stype[s] == T_F2C) H.f2c_ghost.push_back((uint32_t)s);
         }
     }
-    // c2f sources: the covering coarse leaf of every ghost and that leaf's
-    // coarse fluid neighbours along the three axes (across periodic faces),
-    // all captured once per coarse step
+    // c2f sources.  Ghosts are grouped by their covering coarse leaf P (one
+    // group per P, child q = fine-coordinate parity x + 2y + 4z); each group
+    // captures P and P's coarse fluid neighbours -x, +x, -y, +y, -z, +z
+    // (across periodic faces), once per coarse step.
     for (int l = 1; l < NL; ++l) {
         HostLevel& H = S->h[l];
         HostLevel& C = S->h[l - 1];
         const size_t ng = H.c2f_ghost.size();
-        std::vector<int64_t> src(ng * kC2FCaps, -1);
+        std::vector<int64_t> par(ng);
+        std::vector<uint8_t> chq(ng);
         parallel_chunks(ng, 4096, [&](size_t k0, size_t k1) {
             for (size_t k = k0; k < k1; ++k) {
                 const uint32_t g = H.c2f_ghost[k];
                 const size_t b = g / 64;
                 const int loc = (int)(g % 64);
-                const int in = H.binst[b];
+                const int32_t fx = H.bcoord[3 * b] * 4 + (loc & 3), fy = H.bcoord[3 * b + 1] * 4 + ((loc >> 2) & 3),
+                              fz = H.bcoord[3 * b + 2] * 4 + (loc >> 4);
+                par[k] = C.slot_at(H.binst[b], fx >> 1, fy >> 1, fz >> 1);
+                chq[k] = (uint8_t)((fx & 1) | ((fy & 1) << 1) | ((fz & 1) << 2));
+            }
+        });
+        for (size_t k = 0; k < ng; ++k)
+            if (par[k] < 0) return mesh_error("c2f ghost with no covering coarse leaf");
+        std::vector<uint32_t> order(ng);
+        for (size_t k = 0; k < ng; ++k) order[k] = (uint32_t)k;
+        std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return par[x] < par[y]; });
+        std::vector<int64_t> gpar;
+        H.c2f_gchild.clear();
+        for (uint32_t k : order) {
+            if (gpar.empty() || gpar.back() != par[k]) {
+                gpar.push_back(par[k]);
+                H.c2f_gchild.insert(H.c2f_gchild.end(), 8, kNoCap);
+            }
+            H.c2f_gchild[8 * (gpar.size() - 1) + chq[k]] = H.c2f_ghost[k];
+        }
+        const size_t G = gpar.size();
+        std::vector<int64_t> src(G * kC2FCaps, -1);
+        parallel_chunks(G, 4096, [&](size_t k0, size_t k1) {
+            for (size_t k = k0; k < k1; ++k) {
+                const int64_t P = gpar[k];
+                const size_t cb = (size_t)P / 64;
+                const int loc = (int)(P % 64);
+                const int in = C.binst[cb];
                 const fl_lbm_params& p = S->inst[in].p;
-                const int32_t pc[3] = {(H.bcoord[3 * b] * 4 + (loc & 3)) >> 1,
-                                       (H.bcoord[3 * b + 1] * 4 + ((loc >> 2) & 3)) >> 1,
-                                       (H.bcoord[3 * b + 2] * 4 + (loc >> 4)) >> 1};
-                src[k * kC2FCaps] = C.slot_at(in, pc[0], pc[1], pc[2]);
+                const int32_t pc[3] = {C.bcoord[3 * cb] * 4 + (loc & 3), C.bcoord[3 * cb + 1] * 4 + ((loc >> 2) & 3),
+                                       C.bcoord[3 * cb + 2] * 4 + (loc >> 4)};
+                src[k * kC2FCaps] = P;
                 for (int a = 0; a < 3; ++a)
                     for (int d = 0; d < 2; ++d) {
                         int32_t q[3] = {pc[0], pc[1], pc[2]};
@@ -666,8 +697,6 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
                     }
             }
         });
-        for (size_t k = 0; k < ng; ++k)
-            if (src[k * kC2FCaps] < 0) return mesh_error("c2f ghost with no covering coarse leaf");
         std::vector<uint32_t> uniq;
         uniq.reserve(src.size());
         for (int64_t v : src)
@@ -675,10 +704,10 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
         std::sort(uniq.begin(), uniq.end());
         uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
         C.cap_slots = uniq;
-        H.c2f_cap.assign(src.size(), kNoCap);
+        H.c2f_gcap.assign(src.size(), kNoCap);
         for (size_t k = 0; k < src.size(); ++k)
             if (src[k] >= 0)
-                H.c2f_cap[k] = (uint32_t)(std::lower_bound(uniq.begin(), uniq.end(), (uint32_t)src[k]) - uniq.begin());
+                H.c2f_gcap[k] = (uint32_t)(std::lower_bound(uniq.begin(), uniq.end(), (uint32_t)src[k]) - uniq.begin());
     }
     for (int l = 0; l + 1 < NL; ++l) {
         HostLevel& H = S->h[l];
@@ -709,11 +738,11 @@ uint64_t layout_bytes(const fl_lbm_solver* S) {
     for (int l = 0; l < S->L; ++l) {
         const HostLevel& H = S->h[l];
         t += H.nbr.size() * 4 + H.bcoord.size() * 4 + H.binst.size() * 2 + H.bflags.size() * 4 + H.cflags.size() * 8;
-        t += (H.c2f_ghost.size() + H.c2f_cap.size() + H.cap_slots.size() + H.f2c_ghost.size() + H.f2c_child.size() +
+        t += (H.c2f_gchild.size() + H.c2f_gcap.size() + H.cap_slots.size() + H.f2c_ghost.size() + H.f2c_child.size() +
               H.force_links.size() + H.abeg.size() + H.lbeg.size()) * 4;
         t += (uint64_t)NI * (sizeof(float) + sizeof(double));
         t += std::max<uint64_t>((uint64_t)H.nactive * 4, (uint64_t)H.force_links.size() * 3) * 8;
-        t += 2ull * H.nb * 64 * Q * sb + 2ull * H.cap_slots.size() * Q * cb;
+        t += 2ull * H.nb * 64 * Q * sb + 2ull * H.cap_slots.size() * kCapPlanes * cb;
     }
     return t;
 }
@@ -738,8 +767,8 @@ fl_status upload_and_init(fl_lbm_solver* S) {
         if ((r = dev_upload(S, &D.binst, H.binst))) return r;
         if ((r = dev_upload(S, &D.bflags, H.bflags))) return r;
         if ((r = dev_upload(S, &D.cflags, H.cflags))) return r;
-        if ((r = dev_upload(S, &D.c2f_ghost, H.c2f_ghost))) return r;
-        if ((r = dev_upload(S, &D.c2f_cap, H.c2f_cap))) return r;
+        if ((r = dev_upload(S, &D.c2f_gchild, H.c2f_gchild))) return r;
+        if ((r = dev_upload(S, &D.c2f_gcap, H.c2f_gcap))) return r;
         if ((r = dev_upload(S, &D.cap_slots, H.cap_slots))) return r;
         if ((r = dev_upload(S, &D.f2c_ghost, H.f2c_ghost))) return r;
         if ((r = dev_upload(S, &D.f2c_child, H.f2c_child))) return r;
@@ -761,7 +790,7 @@ fl_status upload_and_init(fl_lbm_solver* S) {
         const size_t slots = (size_t)H.nb * 64;
         for (int k = 0; k < 2; ++k) {
             if ((r = dev_alloc(S, &D.buf[k], slots * Q * sb))) return r;
-            if ((r = dev_alloc(S, &D.cap[k], H.cap_slots.size() * Q * cb))) return r;
+            if ((r = dev_alloc(S, &D.cap[k], H.cap_slots.size() * kCapPlanes * cb))) return r;
         }
         D.cur = 0;
         D.ld.nbr = D.nbr;
@@ -807,7 +836,7 @@ fl_status advance(fl_lbm_solver* S, int l) {
         DevLevel& F = S->d[l + 1];
         const HostLevel& HF = S->h[l + 1];
         for (int sub = 0; sub < 2; ++sub) {
-            FL_HIP(launch_fill_c2f<T>(F.ld, F.c2f_ghost, F.c2f_cap, (uint32_t)HF.c2f_ghost.size(), D.cap[0], D.cap[1],
+            FL_HIP(launch_fill_c2f<T>(F.ld, F.c2f_gchild, F.c2f_gcap, (uint32_t)(HF.c2f_gcap.size() / kC2FCaps), D.cap[0], D.cap[1],
                                       ncap, 0.5 * sub, static_cast<T*>(F.buf[F.cur]), st));
             fl_status r = advance<T>(S, l + 1);
             if (r) return r;
@@ -1265,7 +1294,7 @@ fl_status remap(fl_lbm_solver* O, fl_lbm_solver* N) {
         const uint32_t* capsl = l > 0 ? N->d[l - 1].cap_slots : nullptr;
         const T* Af = l + 1 < N->L ? static_cast<const T*>(N->d[l + 1].buf[0]) : nullptr;
         const size_t Sf = l + 1 < N->L ? N->d[l + 1].ld.S : 0;
-        hipError_t e = launch_ghost_eq<T>(D.c2f_ghost, D.c2f_cap, capsl, (uint32_t)H.c2f_ghost.size(), Ac, Sc,
+        hipError_t e = launch_ghost_eq<T>(D.c2f_gchild, D.c2f_gcap, capsl, (uint32_t)(H.c2f_gcap.size() / kC2FCaps), Ac, Sc,
                                           D.f2c_ghost, D.f2c_child, (uint32_t)H.f2c_ghost.size(), Af, Sf,
                                           static_cast<T*>(D.buf[0]), static_cast<T*>(D.buf[1]), D.ld.S, N->stream);
         if (e != hipSuccess) st = fl::hip_fail(e, "launch_ghost_eq", __FILE__, __LINE__);

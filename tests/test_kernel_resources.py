@@ -1,0 +1,54 @@
+"""Register budget of the lattice kernels (compiled here for gfx950, no GPU).
+
+Every hot lattice kernel must run without scratch (no register spills), and
+the collide-stream instances must keep at least four waves per SIMD: a spill
+in the collide turns an HBM-bound kernel into a scratch-bound one (three to
+six times slower), which no parity test notices.  The figures come from the
+compiler's kernel resource-usage remarks for the same flags the library
+build uses."""
+import re
+import subprocess
+
+import pytest
+
+from fluidloom_amd import build
+
+
+@pytest.fixture(scope="module")
+def usage(tmp_path_factory):
+    out = tmp_path_factory.mktemp("res") / "lbm_kernels.o"
+    src = build.CSRC / "lbm_kernels.hip"
+    cmd = [build.HIPCC, *build.COMMON, "-c", str(src), "-o", str(out), "-Rpass-analysis=kernel-resource-usage"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    kernels = {}
+    for block in re.split(r"remark: Function Name: ", r.stderr)[1:]:
+        name = block.split()[0]
+
+        def field(key):
+            m = re.search(re.escape(key) + r": (\d+)", block)
+            return int(m.group(1)) if m else None
+
+        kernels[name] = dict(vgpr=field("VGPRs"), scratch=field("ScratchSize [bytes/lane]"),
+                             occupancy=field("Occupancy [waves/SIMD]"))
+    return kernels
+
+
+def _lattice(usage, stem):
+    return {k: v for k, v in usage.items() if stem in k}
+
+
+@pytest.mark.parametrize("stem", ["k_collide", "k_fill_c2f", "k_fill_f2c", "k_capture"])
+def test_no_scratch(usage, stem):
+    ks = _lattice(usage, stem)
+    assert ks, f"no {stem} instances found"
+    spilled = {k: v for k, v in ks.items() if v["scratch"]}
+    assert not spilled, spilled
+
+
+def test_collide_occupancy(usage):
+    ks = _lattice(usage, "k_collide")
+    # 4 storage types x 13 launch variants
+    assert len(ks) == 52
+    low = {k: v for k, v in ks.items() if v["occupancy"] < 4}
+    assert not low, low
