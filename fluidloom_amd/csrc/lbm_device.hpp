@@ -103,8 +103,13 @@ struct fp16c {
 
 __device__ inline float fp16c_decode(uint16_t c) {
     const uint32_t m = c & 0x7FFFu;
-    const float a = m >= 0x800u ? __uint_as_float((((m >> 11) + 113u) << 23) | ((m & 0x7FFu) << 12))
-                                : (float)m * 5.9604644775390625e-08f;
+    // normal codes: exponent (m >> 11) - 14, mantissa m & 0x7FF, i.e. the bits
+    // (m << 12) + (113 << 23).  Codes below 0x800 are multiples of 2^-24:
+    // form 2^-13 + m 2^-24 with the exponent raised by one and subtract 2^-13
+    // (both steps exact).  No select between two encodings: a per-population
+    // select here made the compiler branch and multiply the collide's code.
+    const uint32_t sub = m < 0x800u ? 1u : 0u;
+    const float a = __uint_as_float((m << 12) + ((113u + sub) << 23)) - (sub ? 1.220703125e-04f : 0.0f);
     return __uint_as_float(__float_as_uint(a) | ((uint32_t)(c & 0x8000u) << 16));
 }
 

@@ -83,11 +83,10 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
         return (unsigned)nb * 64u + (unsigned)((px & 3) + 4 * (py & 3) + 16 * (pz & 3));
     };
     R f[Q];
-    if (std::is_same<T, _Float16>::value && simple) {
-        // FP16 storage, every cell fluid, every link regular: 19 independent
-        // loads back to back.  The other storage types keep the interleaved
-        // form below: it measured faster for FP32, and the FP16C decode in
-        // this form needs more registers than one wave slot holds.
+    if (sizeof(T) == 2 && simple) {
+        // 16-bit storage, every cell fluid, every link regular: 19 independent
+        // loads back to back.  FP32 and FP64 keep the interleaved form below,
+        // which measured faster for FP32.
 #pragma unroll
         for (int i = 0; i < Q; ++i) f[i] = Store<T>::load(A + (size_t)i * S, source(i), i);
     } else {

@@ -52,3 +52,14 @@ def test_collide_occupancy(usage):
     assert len(ks) == 52
     low = {k: v for k, v in ks.items() if v["occupancy"] < 4}
     assert not low, low
+    # the non-persistent FP32 / FP16 / FP16C forms (the ones the autotuner
+    # picks on large levels) keep the full eight waves per SIMD
+    full = {}
+    for k, v in ks.items():
+        m = re.search(r"k_collideI(.+?)Li(\d+)ELi(\d)E", k)
+        assert m, k
+        if m.group(1) != "d" and m.group(3) != "2":
+            full[k] = v["occupancy"]
+    assert len(full) == 30
+    short = {k: o for k, o in full.items() if o < 8}
+    assert not short, short
