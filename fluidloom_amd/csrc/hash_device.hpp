@@ -19,15 +19,33 @@ __device__ inline uint64_t hash_slot(uint64_t key, uint64_t cap) {
     return (key * 0x9e3779b97f4a7c15ull) & (cap - 1);
 }
 
+// Whether slot s holds `key` after the call (claimed now or before); false
+// when another key holds it.  A slot's key never changes once set, so a
+// relaxed load decides every case but an empty slot, and only that one needs
+// the CAS: a key inserted millions of times (a hot duplicate) then costs
+// loads, not serialised atomics on one address.
+__device__ inline bool claim_slot(uint64_t* tkeys, uint64_t s, uint64_t key) {
+    const uint64_t cur = __hip_atomic_load(&tkeys[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur == key) return true;
+    if (cur != 0xFFFFFFFFFFFFFFFFull) return false;
+    const unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&tkeys[s]),
+                                              0xFFFFFFFFFFFFFFFFull, (unsigned long long)key);
+    return prev == 0xFFFFFFFFFFFFFFFFull || prev == key;
+}
+
+// Keeps the smallest value of a slot.  Stored values only decrease, so a
+// loaded value at or below `val` means the atomic can be skipped.
+__device__ inline void keep_min(uint32_t* tvals, uint64_t s, uint32_t val) {
+    if (__hip_atomic_load(&tvals[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > val) atomicMin(&tvals[s], val);
+}
+
 // Returns the number of probes used (kMaxProbes when the key was dropped).
 __device__ inline uint32_t hash_insert(uint64_t* tkeys, uint32_t* tvals, uint64_t cap, uint64_t key,
                                        uint32_t val) {
     uint64_t s = hash_slot(key, cap);
     for (uint32_t p = 0; p < kMaxProbes; ++p) {
-        unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&tkeys[s]),
-                                            0xFFFFFFFFFFFFFFFFull, (unsigned long long)key);
-        if (prev == 0xFFFFFFFFFFFFFFFFull || prev == key) {
-            atomicMin(&tvals[s], val);
+        if (claim_slot(tkeys, s, key)) {
+            keep_min(tvals, s, val);
             return p;
         }
         s = (s + 1) & (cap - 1);
@@ -72,10 +90,8 @@ __device__ inline uint64_t cell_slot(uint64_t key, uint64_t cap) {
 __device__ inline void cell_insert(uint64_t* tkeys, uint32_t* tvals, uint64_t cap, uint64_t key, uint32_t val) {
     uint64_t s = cell_slot(key, cap);
     for (uint64_t p = 0; p < cap; ++p) {
-        unsigned long long prev = atomicCAS(reinterpret_cast<unsigned long long*>(&tkeys[s]),
-                                            0xFFFFFFFFFFFFFFFFull, (unsigned long long)key);
-        if (prev == 0xFFFFFFFFFFFFFFFFull || prev == key) {
-            atomicMin(&tvals[s], val);
+        if (claim_slot(tkeys, s, key)) {
+            keep_min(tvals, s, val);
             return;
         }
         s = (s + 1) & (cap - 1);

@@ -216,3 +216,38 @@ def test_amr_microbenchmark_full_size_properties(fl):
     assert r["balance_converged"]
     assert r["cells_out"] <= r["cells"] + 7 * r["split"]
     assert r["merged_groups"] > 0 and r["split"] > 0
+
+
+def test_hash_empty_build_and_queries(fl, oracle):
+    # an empty table answers INVALID for every key, including the EMPTY sentinel
+    cap = int(fl.lib().fl_hash_capacity(0))
+    assert cap == oracle.hash_capacity(0)
+    tk, tv = fl.DeviceArray(cap, np.uint64), fl.DeviceArray(cap, np.uint32)
+    fl.check(fl.lib().fl_hash_clear(tk.p, tv.p, cap, None))
+    fl.check(fl.lib().fl_hash_build(tk.p, tv.p, cap, None, None, 0, None, None))
+    q = np.array([0, 1, 12345, 0xFFFFFFFFFFFFFFFF], np.uint64)
+    dq = dev(fl, q)
+    out = fl.DeviceArray(len(q), np.uint32)
+    fl.check(fl.lib().fl_hash_query(tk.p, tv.p, cap, dq.p, out.p, len(q), None, None))
+    assert np.all(out.to_host() == 0xFFFFFFFF)
+
+
+def test_regrid_empty_and_single_cell(fl, oracle):
+    from fluidloom_amd import amr
+    g = amr.GpuCells(0)
+    out, st = amr.regrid(g, balance_mode=1, max_split_level=4, root=(1, 1, 1))
+    assert out.n == 0 and st.num_cells_out == 0 and st.balance_converged
+    # one root cell flagged for refinement: eight children in child order
+    one = amr.GpuCells.from_host(x=np.array([0]), y=np.array([0]), z=np.array([0]), level=np.array([0]),
+                                 flag=np.array([1]))
+    out, st = amr.regrid(one, balance_mode=1, max_split_level=4, root=(1, 1, 1))
+    h = out.to_host()
+    assert st.num_split == 1 and out.n == 8
+    q = np.arange(8)
+    assert np.array_equal(h["x"], q & 1) and np.array_equal(h["y"], (q >> 1) & 1)
+    assert np.array_equal(h["z"], (q >> 2) & 1) and np.all(h["level"] == 1)
+    # the eight children flagged -1 merge back into the root cell
+    back = amr.GpuCells.from_host(x=h["x"], y=h["y"], z=h["z"], level=h["level"], flag=-np.ones(8, np.int32))
+    out2, st2 = amr.regrid(back, balance_mode=1, max_split_level=4, root=(1, 1, 1))
+    h2 = out2.to_host()
+    assert st2.num_groups == 1 and out2.n == 1 and h2["level"][0] == 0 and h2["x"][0] == 0
