@@ -366,25 +366,30 @@ __global__ __launch_bounds__(kScanThreads) void k_compact(CellsC c, size_t n, co
                                                           CellsW out, size_t cap, const float* __restrict__ fin,
                                                           float* __restrict__ fout, int ncomp,
                                                           uint64_t* __restrict__ ws, uint64_t* __restrict__ nkeep) {
-    __shared__ ScanShared sh;
-    const uint32_t tile = scan_tile_id(sh, ws);
-    const size_t base = (size_t)tile * kScanTile + (size_t)threadIdx.x * kScanItems;
+    // striped tile: element j of this thread is t0 + j * kScanThreads + tid,
+    // so each round's loads and the kept elements' stores are coalesced
+    __shared__ StripedShared sh;
+    const uint32_t tile = scan_tile_id(sh.sc, ws);
+    const size_t t0 = (size_t)tile * kScanTile + threadIdx.x;
     uint32_t kmask = 0;
+#pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
-        size_t i = base + j;
+        const size_t i = t0 + (size_t)j * kScanThreads;
         if (i < n && !(c.flag[i] > 0) && (group_id ? group_id[i] == kInvalid : true)) kmask |= 1u << j;
     }
-    uint64_t o = scan_tile_prefix(sh, ws, tile, (uint64_t)__popc(kmask), nkeep, n);
+    uint32_t rel[kScanItems];
+    const uint64_t base = striped_positions(sh, ws, tile, kmask, nkeep, n, rel);
+#pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
         if (!(kmask & (1u << j))) continue;
-        size_t i = base + j;
+        const size_t i = t0 + (size_t)j * kScanThreads;
+        const uint64_t o = base + rel[j];
         if (o < cap) {
             out.x[o] = c.x[i]; out.y[o] = c.y[i]; out.z[o] = c.z[i];
             out.level[o] = c.level[i]; out.state[o] = c.state[i];
             out.material[o] = c.material[i]; out.flag[o] = 0;
             for (int k = 0; k < ncomp; ++k) fout[o * ncomp + k] = fin[i * ncomp + k];
         }
-        ++o;
     }
 }
 
@@ -398,23 +403,30 @@ __global__ __launch_bounds__(kScanThreads) void k_split(CellsC c, size_t n, int 
                                                         const float* __restrict__ fin, float* __restrict__ fout,
                                                         int ncomp, uint64_t* __restrict__ ws,
                                                         uint64_t* __restrict__ nsplit) {
-    __shared__ ScanShared sh;
-    const uint32_t tile = scan_tile_id(sh, ws);
-    const size_t base = (size_t)tile * kScanTile + (size_t)threadIdx.x * kScanItems;
+    // striped tile (element j of this thread is t0 + j * kScanThreads + tid):
+    // coalesced predicate loads; split parents of one round write their
+    // children as consecutive 8-cell blocks
+    __shared__ StripedShared sh;
+    const uint32_t tile = scan_tile_id(sh.sc, ws);
+    const size_t t0 = (size_t)tile * kScanTile + threadIdx.x;
     uint32_t smask = 0;
+#pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
-        size_t i = base + j;
+        const size_t i = t0 + (size_t)j * kScanThreads;
         if (i < n && c.flag[i] > 0 && c.state[i] == 0 && c.level[i] < max_level) smask |= 1u << j;
     }
-    uint64_t r = scan_tile_prefix(sh, ws, tile, (uint64_t)__popc(smask), nsplit, n);
+    uint32_t rel[kScanItems];
+    const uint64_t rbase = striped_positions(sh, ws, tile, smask, nsplit, n, rel);
     const uint64_t off = out_off ? *out_off : 0;
+#pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
-        size_t i = base + j;
+        const size_t i = t0 + (size_t)j * kScanThreads;
         if (i >= n) break;
         if (!(smask & (1u << j))) {
             if (child_start) child_start[i] = kInvalid;
             continue;
         }
+        const uint64_t r = rbase + rel[j];
         if (child_start) child_start[i] = (uint32_t)(8 * r);
         for (int q = 0; q < 8; ++q) {
             uint64_t o = off + 8 * r + q;
@@ -428,7 +440,6 @@ __global__ __launch_bounds__(kScanThreads) void k_split(CellsC c, size_t n, int 
             out.flag[o] = 0;
             for (int k = 0; k < ncomp; ++k) fout[o * ncomp + k] = fin[i * ncomp + k];
         }
-        ++r;
     }
 }
 
