@@ -379,11 +379,26 @@ __global__ __launch_bounds__(kScanThreads) void k_compact(CellsC c, size_t n, co
     }
     uint32_t rel[kScanItems];
     const uint64_t base = striped_positions(sh, ws, tile, kmask, nkeep, n, rel);
-#pragma unroll
+    // positions recomputed round by round (ballot rank + per-wave counts), so
+    // the rounds run as a loop rather than unrolled with all loads in flight
+    const int w = threadIdx.x >> 6;
+    const uint64_t below = lanemask_lt();
+    uint64_t run = base;
+#pragma unroll 1
     for (int j = 0; j < kScanItems; ++j) {
-        if (!(kmask & (1u << j))) continue;
+        const bool mine = (kmask >> j) & 1u;
+        const uint64_t bal = __ballot(mine);
+        uint32_t cnt = 0, before = 0;
+#pragma unroll
+        for (int v = 0; v < kScanThreads / 64; ++v) {
+            const uint32_t cv = sh.cnt[j][v];
+            if (v < w) before += cv;
+            cnt += cv;
+        }
+        const uint64_t o = run + before + (uint32_t)__popcll(bal & below);
+        run += cnt;
+        if (!mine) continue;
         const size_t i = t0 + (size_t)j * kScanThreads;
-        const uint64_t o = base + rel[j];
         if (o < cap) {
             out.x[o] = c.x[i]; out.y[o] = c.y[i]; out.z[o] = c.z[i];
             out.level[o] = c.level[i]; out.state[o] = c.state[i];
@@ -418,28 +433,59 @@ __global__ __launch_bounds__(kScanThreads) void k_split(CellsC c, size_t n, int 
     uint32_t rel[kScanItems];
     const uint64_t rbase = striped_positions(sh, ws, tile, smask, nsplit, n, rel);
     const uint64_t off = out_off ? *out_off : 0;
-#pragma unroll
+    // round by round: the round's split parents go to LDS in rank order, then
+    // the whole block writes their 8 * count children with consecutive lanes
+    // on consecutive cells
+    __shared__ int32_t px[kScanThreads], py[kScanThreads], pz[kScanThreads];
+    __shared__ uint32_t pmat[kScanThreads], pidx_[kScanThreads];
+    __shared__ uint8_t plev[kScanThreads], pst[kScanThreads];
+    const int w = threadIdx.x >> 6;
+    const uint64_t below = lanemask_lt();
+    uint32_t run = 0;  // kept elements of the earlier rounds
+#pragma unroll 1
     for (int j = 0; j < kScanItems; ++j) {
         const size_t i = t0 + (size_t)j * kScanThreads;
-        if (i >= n) break;
-        if (!(smask & (1u << j))) {
-            if (child_start) child_start[i] = kInvalid;
-            continue;
+        const bool mine = (smask >> j) & 1u;
+        const uint64_t bal = __ballot(mine);
+        uint32_t cnt = 0, before = 0;
+#pragma unroll
+        for (int v = 0; v < kScanThreads / 64; ++v) {
+            const uint32_t cv = sh.cnt[j][v];
+            if (v < w) before += cv;
+            cnt += cv;
         }
-        const uint64_t r = rbase + rel[j];
-        if (child_start) child_start[i] = (uint32_t)(8 * r);
-        for (int q = 0; q < 8; ++q) {
-            uint64_t o = off + 8 * r + q;
+        if (i < n) {
+            if (mine) {
+                const uint32_t k = before + (uint32_t)__popcll(bal & below);
+                px[k] = c.x[i];
+                py[k] = c.y[i];
+                pz[k] = c.z[i];
+                plev[k] = c.level[i];
+                pst[k] = c.state[i];
+                pmat[k] = c.material[i];
+                pidx_[k] = (uint32_t)i;
+                if (child_start) child_start[i] = (uint32_t)(8 * (rbase + run + k));
+            } else if (child_start) {
+                child_start[i] = kInvalid;
+            }
+        }
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < 8 * cnt; k += kScanThreads) {
+            const uint32_t p = k >> 3;
+            const int q = (int)(k & 7u);
+            const uint64_t o = off + 8 * (rbase + run) + k;
             if (o >= cap) continue;
-            out.x[o] = (c.x[i] << 1) | (q & 1);
-            out.y[o] = (c.y[i] << 1) | ((q >> 1) & 1);
-            out.z[o] = (c.z[i] << 1) | ((q >> 2) & 1);
-            out.level[o] = (uint8_t)(c.level[i] + 1);
-            out.state[o] = c.state[i];
-            out.material[o] = c.material[i];
+            out.x[o] = (px[p] << 1) | (q & 1);
+            out.y[o] = (py[p] << 1) | ((q >> 1) & 1);
+            out.z[o] = (pz[p] << 1) | ((q >> 2) & 1);
+            out.level[o] = (uint8_t)(plev[p] + 1);
+            out.state[o] = pst[p];
+            out.material[o] = pmat[p];
             out.flag[o] = 0;
-            for (int k = 0; k < ncomp; ++k) fout[o * ncomp + k] = fin[i * ncomp + k];
+            for (int m = 0; m < ncomp; ++m) fout[o * ncomp + m] = fin[(size_t)pidx_[p] * ncomp + m];
         }
+        run += cnt;
+        __syncthreads();
     }
 }
 
