@@ -3,7 +3,8 @@ codec sources (src/core/hilbert/HilbertCodec.cpp and
 src/adaptation/utils/HilbertCodec3D.cpp), compiled in place by
 `make -C oracle ref` into oracle/_ref/libref_hilbert.so.
 
-Run in the build container (the reference tree is not on the GPU box):
+Run where the reference tree is present (the fixture it writes is what the
+tests read elsewhere):
     python tests/golden/make_hilbert_golden.py
 """
 import ctypes as C
