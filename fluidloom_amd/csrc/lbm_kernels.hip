@@ -49,8 +49,8 @@ __device__ inline R inv_tau_of(const LevelDev& L, int inst) {
 // One wave per brick, lane = cell.  The brick's 27 neighbour entries are
 // wave-uniform loads (scalar registers); for direction i a lane's source
 // brick is one of at most four of them, chosen by whether the pulled
-// position leaves the brick along each axis.  Offsets inside a population
-// plane are 32-bit (the layout keeps 19 * slots below 2^32), on a
+// position leaves the brick along each axis.  Offsets from a plane's base
+// are 32-bit (below 8 * slots, and the layout keeps slots below 2^27), on a
 // wave-uniform plane base.
 template <typename T>
 __device__ __attribute__((always_inline)) inline void collide_brick(const LevelDev& L, const T* __restrict__ A,
@@ -71,7 +71,7 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
     const bool active = simple || (w & kActiveBit);
     const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
     const size_t S = L.S;
-    // pulled source of direction i: slot inside plane i (A + i*S)
+    // pulled source of direction i: offset from plane i's base (A + plane_base)
     auto source = [&](int i) -> unsigned {
         const int px = lx - cx(i), py = ly - cy(i), pz = lz - cz(i);
         const bool ex = (unsigned)px > 3u, ey = (unsigned)py > 3u, ez = (unsigned)pz > 3u;
@@ -80,7 +80,7 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
         const int n001 = NB[13 + kz], n101 = NB[13 + kx + kz], n011 = NB[13 + ky + kz], n111 = NB[13 + kx + ky + kz];
         const int nb = ez ? (ey ? (ex ? n111 : n011) : (ex ? n101 : n001))
                           : (ey ? (ex ? n110 : n010) : (ex ? n100 : n000));
-        return (unsigned)nb * 64u + (unsigned)((px & 3) + 4 * (py & 3) + 16 * (pz & 3));
+        return (unsigned)nb * (unsigned)brick_stride(i) + (unsigned)((px & 3) + 4 * (py & 3) + 16 * (pz & 3));
     };
     R f[Q];
     if (sizeof(T) == 2 && simple) {
@@ -88,7 +88,7 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
         // loads back to back.  FP32 and FP64 keep the interleaved form below,
         // which measured faster for FP32.
 #pragma unroll
-        for (int i = 0; i < Q; ++i) f[i] = Store<T>::load(A + (size_t)i * S, source(i), i);
+        for (int i = 0; i < Q; ++i) f[i] = Store<T>::load(A + plane_base(i, S), source(i), i);
     } else {
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
@@ -96,7 +96,7 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
             if (active) {
                 const unsigned code = (unsigned)(w >> (2 * i)) & 3u;
                 if (code == kLinkRegular)
-                    f[i] = Store<T>::load(A + (size_t)i * S, source(i), i);
+                    f[i] = Store<T>::load(A + plane_base(i, S), source(i), i);
                 else
                     f[i] = special_link<T, R>(L, A, b, slot, i, code, lx - cx(i), ly - cy(i), lz - cz(i));
             }
@@ -114,12 +114,12 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
                                     ((uint32_t)L.level << 24) ^ (*L.tick * 0x27D4EB2Fu))));
 #pragma unroll
         for (int i = 0; i < Q; ++i)
-            Store<T>::store_sr(B + (size_t)i * S, slot, i, f[i],
+            Store<T>::store_sr(B + plane_base(i, S), (unsigned)b * (unsigned)brick_stride(i) + (unsigned)lane, i, f[i],
                                mix32(seed + 0x9E3779B9u * (uint32_t)(i + 1)));
     } else {
 #pragma unroll
         for (int i = 0; i < Q; ++i)
-            Store<T>::store_nt(B + (size_t)i * S, slot, i, f[i]);
+            Store<T>::store_nt(B + plane_base(i, S), (unsigned)b * (unsigned)brick_stride(i) + (unsigned)lane, i, f[i]);
     }
 }
 

@@ -1581,10 +1581,11 @@ fl_status fl_lbm_set_live(fl_lbm_solver* S, int inst, int live) {
             DevLevel& D = S->d[l];
             const size_t b0 = H.abeg[inst], b1 = H.abeg[inst + 1];
             if (b1 == b0) continue;
-            for (int q = 0; q < Q; ++q) {
-                const size_t off = ((size_t)q * D.ld.S + b0 * 64) * sb;
+            // the instance's active bricks are one run in every plane group
+            for (int q = 0; q < Q; q += kPlaneGroup) {
+                const size_t off = (plane_base(q, D.ld.S) + b0 * brick_stride(q)) * sb;
                 FL_HIP(hipMemcpyAsync(static_cast<char*>(D.buf[1 - D.cur]) + off,
-                                      static_cast<const char*>(D.buf[D.cur]) + off, (b1 - b0) * 64 * sb,
+                                      static_cast<const char*>(D.buf[D.cur]) + off, (b1 - b0) * brick_stride(q) * sb,
                                       hipMemcpyDeviceToDevice, S->stream));
             }
         }
