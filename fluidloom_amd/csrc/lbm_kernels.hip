@@ -110,12 +110,18 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
         const int32_t* c3 = L.bcoord + 3 * b;
         const uint32_t gx = (uint32_t)(c3[0] * 4 + lx), gy = (uint32_t)(c3[1] * 4 + ly),
                        gz = (uint32_t)(c3[2] * 4 + lz);
-        const uint32_t seed = mix32(gx * 0x9E3779B1u ^ mix32(gy * 0x85EBCA77u ^ mix32(gz * 0xC2B2AE3Du ^
-                                    ((uint32_t)L.level << 24) ^ (*L.tick * 0x27D4EB2Fu))));
+        // one hash per cell, then a xorshift32 stream for the 19 directions
+        // (shifts and xors only: the kernel is bound by VALU issue in 16-bit
+        // storage, and a per-direction hash costs two quarter-rate multiplies)
+        uint32_t x = mix32(gx * 0x9E3779B1u ^ mix32(gy * 0x85EBCA77u ^ mix32(gz * 0xC2B2AE3Du ^
+                               ((uint32_t)L.level << 24) ^ (*L.tick * 0x27D4EB2Fu)))) | 1u;
 #pragma unroll
-        for (int i = 0; i < Q; ++i)
-            Store<T>::store_sr(B + plane_base(i, S), (unsigned)b * (unsigned)brick_stride(i) + (unsigned)lane, i, f[i],
-                               mix32(seed + 0x9E3779B9u * (uint32_t)(i + 1)));
+        for (int i = 0; i < Q; ++i) {
+            x ^= x << 13;
+            x ^= x >> 17;
+            x ^= x << 5;
+            Store<T>::store_sr(B + plane_base(i, S), (unsigned)b * (unsigned)brick_stride(i) + (unsigned)lane, i, f[i], x);
+        }
     } else {
 #pragma unroll
         for (int i = 0; i < Q; ++i)
