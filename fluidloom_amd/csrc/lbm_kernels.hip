@@ -189,7 +189,8 @@ __device__ inline double f2c_factor(double tau0, int l) {
 // non-equilibrium part.
 template <typename T>
 __global__ __launch_bounds__(kWG) void k_fill_c2f(LevelDev Lf, const uint32_t* __restrict__ gchild,
-                                                  const uint32_t* __restrict__ gcap, uint32_t ngroups,
+                                                  const uint32_t* __restrict__ gcap,
+                                                  const uint32_t* __restrict__ gmask, uint32_t ngroups,
                                                   const typename Store<T>::R* __restrict__ cap0,
                                                   const typename Store<T>::R* __restrict__ cap1, uint32_t ncap,
                                                   typename Store<T>::R alpha, T* __restrict__ A) {
@@ -200,12 +201,13 @@ __global__ __launch_bounds__(kWG) void k_fill_c2f(LevelDev Lf, const uint32_t* _
     // every lane of a group shares k, so a group is wholly in or out
     const bool on = k < ngroups;
     uint32_t ci[kC2FCaps];
-    uint32_t g = kNoCap;
+    uint32_t g = kNoCap, want = 0;
     R pv[3] = {0, 0, 0}, mv[3] = {0, 0, 0};
     if (on) {
 #pragma unroll
         for (int j = 0; j < kC2FCaps; ++j) ci[j] = gcap[(size_t)kC2FCaps * k + j];
         g = gchild[(size_t)8 * k + q];
+        want = gmask[(size_t)8 * k + q];
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             const int i = q + 8 * r;  // population i of P
@@ -261,8 +263,10 @@ __global__ __launch_bounds__(kWG) void k_fill_c2f(LevelDev Lf, const uint32_t* _
     }
     const R usq = phi[1] * phi[1] + phi[2] * phi[2] + phi[3] * phi[3];
     const R usp = mp[1] * mp[1] + mp[2] * mp[2] + mp[3] * mp[3];
+    // only the populations some fluid leaf pulls from this ghost
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
+        if (!((want >> i) & 1u)) continue;
         const R e = feq(i, phi[0], phi[1], phi[2], phi[3], usq);
         Store<T>::store(A, pidx(i, Lf.S, g), i, e + fac * (f[i] - feq(i, mp[0], mp[1], mp[2], mp[3], usp)));
     }
@@ -775,13 +779,14 @@ hipError_t launch_capture(const LevelDev& L, const T* A, const uint32_t* slots, 
 }
 
 template <typename T>
-hipError_t launch_fill_c2f(const LevelDev& Lf, const uint32_t* gchild, const uint32_t* gcap, uint32_t ngroups,
-                           const void* cap0, const void* cap1, uint32_t ncap, double alpha, T* A, hipStream_t s) {
+hipError_t launch_fill_c2f(const LevelDev& Lf, const uint32_t* gchild, const uint32_t* gcap, const uint32_t* gmask,
+                           uint32_t ngroups, const void* cap0, const void* cap1, uint32_t ncap, double alpha, T* A,
+                           hipStream_t s) {
     if (ngroups == 0) return hipSuccess;
     using R = typename Store<T>::R;
     // eight lanes per group, one pass (no grid stride: a group must stay whole)
     const size_t blocks = ((size_t)ngroups * 8 + kWG - 1) / kWG;
-    hipLaunchKernelGGL(k_fill_c2f<T>, dim3((unsigned)blocks), dim3(kWG), 0, s, Lf, gchild, gcap, ngroups,
+    hipLaunchKernelGGL(k_fill_c2f<T>, dim3((unsigned)blocks), dim3(kWG), 0, s, Lf, gchild, gcap, gmask, ngroups,
                        static_cast<const R*>(cap0), static_cast<const R*>(cap1), ncap, (R)alpha, A);
     return hipGetLastError();
 }
@@ -887,8 +892,8 @@ hipError_t launch_ghost_eq(const uint32_t* c2f_gchild, const uint32_t* c2f_gcap,
 #define FL_INST(T)                                                                                            \
     template hipError_t launch_collide<T>(const LevelDev&, const T*, T*, int, hipStream_t);                    \
     template hipError_t launch_capture<T>(const LevelDev&, const T*, const uint32_t*, uint32_t, void*, hipStream_t); \
-    template hipError_t launch_fill_c2f<T>(const LevelDev&, const uint32_t*, const uint32_t*, uint32_t, const void*, \
-                                           const void*, uint32_t, double, T*, hipStream_t);                     \
+    template hipError_t launch_fill_c2f<T>(const LevelDev&, const uint32_t*, const uint32_t*, const uint32_t*,  \
+                                           uint32_t, const void*, const void*, uint32_t, double, T*, hipStream_t); \
     template hipError_t launch_fill_f2c<T>(const LevelDev&, const uint32_t*, const uint32_t*, uint32_t,         \
                                            const LevelDev&, const T*, T*, hipStream_t);                         \
     template hipError_t launch_init<T>(const LevelDev&, T*, T*, hipStream_t);                                   \

@@ -21,8 +21,9 @@ hipError_t launch_tick(uint32_t* tick, hipStream_t s);
 template <typename T>
 hipError_t launch_capture(const LevelDev& L, const T* A, const uint32_t* slots, uint32_t ncap, void* cap, hipStream_t s);
 template <typename T>
-hipError_t launch_fill_c2f(const LevelDev& Lf, const uint32_t* gchild, const uint32_t* gcap, uint32_t ngroups,
-                           const void* cap0, const void* cap1, uint32_t ncap, double alpha, T* A, hipStream_t s);
+hipError_t launch_fill_c2f(const LevelDev& Lf, const uint32_t* gchild, const uint32_t* gcap, const uint32_t* gmask,
+                           uint32_t ngroups, const void* cap0, const void* cap1, uint32_t ncap, double alpha, T* A,
+                           hipStream_t s);
 template <typename T>
 hipError_t launch_fill_f2c(const LevelDev& Lc, const uint32_t* ghost, const uint32_t* child8, uint32_t n,
                            const LevelDev& Lf, const T* Af, T* Ac, hipStream_t s);

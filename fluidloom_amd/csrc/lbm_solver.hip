@@ -154,11 +154,12 @@ struct HostLevel {
     std::vector<uint16_t> binst;
     std::vector<uint32_t> bflags;
     std::vector<uint64_t> cflags;
-    std::vector<uint8_t> stype, needed;
+    std::vector<uint8_t> stype;
+    std::vector<uint32_t> needed;  // per ghost slot: bit i set when a fluid leaf pulls population i from it
     std::vector<uint32_t> c2f_ghost, cap_slots, f2c_ghost, f2c_child, force_links;
     // c2f ghosts grouped by covering coarse leaf: 8 child slots per group
     // (kNoCap where that child is not a ghost) and 7 capture indices
-    std::vector<uint32_t> c2f_gchild, c2f_gcap;
+    std::vector<uint32_t> c2f_gchild, c2f_gcap, c2f_gmask;
     std::vector<uint32_t> abeg;  // [ninst+1] active-brick range of each instance
     std::vector<uint32_t> lbeg;  // [ninst+1] solid-link range of each instance
 
@@ -190,6 +191,7 @@ struct DevLevel {
     uint64_t* cflags = nullptr;
     uint32_t* c2f_gchild = nullptr;
     uint32_t* c2f_gcap = nullptr;
+    uint32_t* c2f_gmask = nullptr;
     uint32_t* cap_slots = nullptr;
     uint32_t* f2c_ghost = nullptr;
     uint32_t* f2c_child = nullptr;
@@ -585,7 +587,7 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
                             uint8_t t2 = s2 < 0 ? T_NONE : H.stype[(size_t)s2];
                             if (regular(t2)) {
                                 code = kLinkOutflow;
-                                if (t2 != T_FLUID) __atomic_store_n(&H.needed[(size_t)s2], (uint8_t)1, __ATOMIC_RELAXED);
+                                if (t2 != T_FLUID) __atomic_fetch_or(&H.needed[(size_t)s2], 1u << i, __ATOMIC_RELAXED);
                             } else if (t2 == T_SOLID || t2 == T_WALL) {
                                 code = kLinkSolid;
                             } else {
@@ -600,7 +602,7 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
                         uint8_t t2 = s2 < 0 ? T_NONE : H.stype[(size_t)s2];
                         if (regular(t2)) {
                             code = kLinkRegular;
-                            if (t2 != T_FLUID) __atomic_store_n(&H.needed[(size_t)s2], (uint8_t)1, __ATOMIC_RELAXED);
+                            if (t2 != T_FLUID) __atomic_fetch_or(&H.needed[(size_t)s2], 1u << i, __ATOMIC_RELAXED);
                         } else if (t2 == T_SOLID || t2 == T_WALL) {
                             code = kLinkSolid;
                             blinks[b].push_back((uint32_t)(s * 32 + i));
@@ -664,12 +666,15 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
         std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return par[x] < par[y]; });
         std::vector<int64_t> gpar;
         H.c2f_gchild.clear();
+        H.c2f_gmask.clear();
         for (uint32_t k : order) {
             if (gpar.empty() || gpar.back() != par[k]) {
                 gpar.push_back(par[k]);
                 H.c2f_gchild.insert(H.c2f_gchild.end(), 8, kNoCap);
+                H.c2f_gmask.insert(H.c2f_gmask.end(), 8, 0u);
             }
             H.c2f_gchild[8 * (gpar.size() - 1) + chq[k]] = H.c2f_ghost[k];
+            H.c2f_gmask[8 * (gpar.size() - 1) + chq[k]] = H.needed[H.c2f_ghost[k]];
         }
         const size_t G = gpar.size();
         std::vector<int64_t> src(G * kC2FCaps, -1);
@@ -738,7 +743,8 @@ uint64_t layout_bytes(const fl_lbm_solver* S) {
     for (int l = 0; l < S->L; ++l) {
         const HostLevel& H = S->h[l];
         t += H.nbr.size() * 4 + H.bcoord.size() * 4 + H.binst.size() * 2 + H.bflags.size() * 4 + H.cflags.size() * 8;
-        t += (H.c2f_gchild.size() + H.c2f_gcap.size() + H.cap_slots.size() + H.f2c_ghost.size() + H.f2c_child.size() +
+        t += (H.c2f_gchild.size() + H.c2f_gcap.size() + H.c2f_gmask.size() + H.cap_slots.size() + H.f2c_ghost.size() +
+              H.f2c_child.size() +
               H.force_links.size() + H.abeg.size() + H.lbeg.size()) * 4;
         t += (uint64_t)NI * (sizeof(float) + sizeof(double));
         t += std::max<uint64_t>((uint64_t)H.nactive * 4, (uint64_t)H.force_links.size() * 3) * 8;
@@ -769,6 +775,7 @@ fl_status upload_and_init(fl_lbm_solver* S) {
         if ((r = dev_upload(S, &D.cflags, H.cflags))) return r;
         if ((r = dev_upload(S, &D.c2f_gchild, H.c2f_gchild))) return r;
         if ((r = dev_upload(S, &D.c2f_gcap, H.c2f_gcap))) return r;
+        if ((r = dev_upload(S, &D.c2f_gmask, H.c2f_gmask))) return r;
         if ((r = dev_upload(S, &D.cap_slots, H.cap_slots))) return r;
         if ((r = dev_upload(S, &D.f2c_ghost, H.f2c_ghost))) return r;
         if ((r = dev_upload(S, &D.f2c_child, H.f2c_child))) return r;
@@ -836,7 +843,7 @@ fl_status advance(fl_lbm_solver* S, int l) {
         DevLevel& F = S->d[l + 1];
         const HostLevel& HF = S->h[l + 1];
         for (int sub = 0; sub < 2; ++sub) {
-            FL_HIP(launch_fill_c2f<T>(F.ld, F.c2f_gchild, F.c2f_gcap, (uint32_t)(HF.c2f_gcap.size() / kC2FCaps), D.cap[0], D.cap[1],
+            FL_HIP(launch_fill_c2f<T>(F.ld, F.c2f_gchild, F.c2f_gcap, F.c2f_gmask, (uint32_t)(HF.c2f_gcap.size() / kC2FCaps), D.cap[0], D.cap[1],
                                       ncap, 0.5 * sub, static_cast<T*>(F.buf[F.cur]), st));
             fl_status r = advance<T>(S, l + 1);
             if (r) return r;
