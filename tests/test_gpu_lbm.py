@@ -157,3 +157,29 @@ def test_couette_exact_across_refined_band(fl, oracle, storage, tol):
     exact = U * ((z + 0.5) / 2.0 ** lv) / N
     assert np.abs(u[:, 0] - exact).max() / U < tol
     g.close()
+
+
+def test_cavity_full_reference_case_matches_oracle(fl, oracle):
+    """The reference's CPU-runnable case at its own size: 32^3 lid-driven
+    cavity, lid 0.05, tau 0.6, one extra level where |w| > 0.5 max, regrid
+    every 50 steps, 500 steps.  Every regrid gives the oracle's mesh bit for
+    bit and the FP64 state ends within 1e-10 of the oracle's."""
+    from fluidloom_amd.cavity import Cavity
+    from tests.cavity_util import OracleCavity
+    g = Cavity(n0=32, max_level=1, storage="f64")
+    o = OracleCavity(oracle, n0=32, max_level=1)
+    for chunk in range(10):
+        g.run(50, 0)
+        o.run(50, 0)
+        if chunk < 9:
+            g.regrid()
+            o.regrid()
+            gs, os_ = g.solver, o.s
+            ka = np.lexsort((gs.x, gs.y, gs.z, gs.level))
+            kb = np.lexsort((os_.x, os_.y, os_.z, os_.level))
+            for a, b in [(gs.x, os_.x), (gs.y, os_.y), (gs.z, os_.z), (gs.level, os_.level)]:
+                assert np.array_equal(a[ka], b[kb])
+    fo = o.s.get_f()
+    fg = g.solver.get_f(o.s.fx, o.s.fy, o.s.fz, o.s.flv)
+    assert np.abs(fg - fo).max() / np.abs(fo).max() < 1e-10
+    assert (g.solver.level == 1).any() and g.regrids == 9
