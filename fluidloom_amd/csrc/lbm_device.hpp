@@ -9,13 +9,15 @@
 // directions in opposite pairs.  Weights 1/3, 1/18, 1/36; cs^2 = 1/3.
 //
 // Storage: slot = brick * 64 + local, local = lx + 4 ly + 16 lz inside a
-// 4x4x4 brick; populations are 64-cell brick planes in plane groups (pidx).  Three storage types:
-// float, double, and _Float16 holding f_q - w_q.
+// 4x4x4 brick; populations are 64-cell brick planes in plane groups (pidx).  Four storage types:
+// float, double, _Float16 holding f_q - w_q, and fp16c holding (f_q - w_q) / w_q
+// (fp16c_codec.hpp).
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 
+#include "fp16c_codec.hpp"
 #include "lattice_gen.hpp"
 
 namespace fl {
@@ -129,60 +131,25 @@ struct Store<_Float16> {
     }
 };
 
-// FP16C: 16-bit storage of x = (f - w_q) / w_q with one sign bit, a 4-bit
-// exponent (bias 14) and an 11-bit mantissa: one more mantissa bit than
-// IEEE binary16 for |x| < 4 (values beyond saturate), subnormal spacing 2^-24
-// below 2^-13.  Low-Mach populations stay well inside |x| < 1.
-struct fp16c {
-    uint16_t bits;
-};
-
-__device__ inline float fp16c_decode(uint16_t c) {
-    const uint32_t m = c & 0x7FFFu;
-    // normal codes: exponent (m >> 11) - 14, mantissa m & 0x7FF, i.e. the bits
-    // (m << 12) + (113 << 23).  Codes below 0x800 are multiples of 2^-24:
-    // form 2^-13 + m 2^-24 with the exponent raised by one and subtract 2^-13
-    // (both steps exact).  No select between two encodings: a per-population
-    // select here made the compiler branch and multiply the collide's code.
-    const uint32_t sub = m < 0x800u ? 1u : 0u;
-    const float a = __uint_as_float((m << 12) + ((113u + sub) << 23)) - (sub ? 1.220703125e-04f : 0.0f);
-    return __uint_as_float(__float_as_uint(a) | ((uint32_t)(c & 0x8000u) << 16));
-}
-
-// rnd: random bits for stochastic rounding (the low 12 are used), or round
-// to nearest even when stochastic is false
-__device__ inline uint16_t fp16c_encode(float x, uint32_t rnd, bool stochastic) {
-    const uint32_t b = __float_as_uint(x);
-    const uint32_t sign = (b >> 16) & 0x8000u;
-    const uint32_t mag = b & 0x7FFFFFFFu;
-    uint32_t code;
-    if (mag >= 0x39000000u) {  // normal range (|x| >= 2^-13)
-        const uint32_t add = stochastic ? (rnd & 0xFFFu) : 0x7FFu + ((mag >> 12) & 1u);
-        const uint32_t m2 = mag + add;
-        code = (mag >= 0x7F800000u || m2 >= 0x40800000u) ? 0x7FFFu : (m2 - (113u << 23)) >> 12;
-    } else {
-        const float a = __uint_as_float(mag) * 16777216.0f;  // 2^24
-        code = stochastic ? (uint32_t)(a + (float)(rnd & 0xFFFu) * (1.0f / 4096.0f)) : (uint32_t)rintf(a);
-    }
-    return (uint16_t)(sign | code);
-}
-
 template <>
 struct Store<fp16c> {
     using R = float;
+    // f = w + w x: the code float times 2^113 w in one fma (exact scaling)
     __device__ static inline float load(const fp16c* p, size_t i, int q) {
         const float w = (float)wq(q);
-        return fmaf(fp16c_decode(p[i].bits), w, w);
+        return fmaf(fp16c_code_float(p[i].bits), w * kFp16cUp, w);
     }
-    __device__ static inline uint16_t enc(float v, int q, uint32_t r, bool sr) {
-        return fp16c_encode((v - (float)wq(q)) * (float)(1.0 / wq(q)), r, sr);
+    __device__ static inline uint16_t enc(float v, int q) {
+        return fp16c_encode_rne((v - (float)wq(q)) * (float)(1.0 / wq(q)));
     }
-    __device__ static inline void store(fp16c* p, size_t i, int q, float v) { p[i].bits = enc(v, q, 0, false); }
+    __device__ static inline void store(fp16c* p, size_t i, int q, float v) { p[i].bits = enc(v, q); }
     __device__ static inline void store_nt(fp16c* p, size_t i, int q, float v) {
-        __builtin_nontemporal_store(enc(v, q, 0, false), &p[i].bits);
+        __builtin_nontemporal_store(enc(v, q), &p[i].bits);
     }
+    // (f - w) (2^-113 / w): for |x| >= 2^-13 the same rounding as (f - w)(1 / w)
     __device__ static inline void store_sr(fp16c* p, size_t i, int q, float v, uint32_t r) {
-        __builtin_nontemporal_store(enc(v, q, r, true), &p[i].bits);
+        const float xs = (v - (float)wq(q)) * ((float)(1.0 / wq(q)) * kFp16cDown);
+        __builtin_nontemporal_store(fp16c_encode_sr_scaled(xs, r), &p[i].bits);
     }
 };
 
