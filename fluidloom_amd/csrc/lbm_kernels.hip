@@ -80,7 +80,9 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
         const int n001 = NB[13 + kz], n101 = NB[13 + kx + kz], n011 = NB[13 + ky + kz], n111 = NB[13 + kx + ky + kz];
         const int nb = ez ? (ey ? (ex ? n111 : n011) : (ex ? n101 : n001))
                           : (ey ? (ex ? n110 : n010) : (ex ? n100 : n000));
-        return (unsigned)nb * (unsigned)brick_stride(i) + (unsigned)((px & 3) + 4 * (py & 3) + 16 * (pz & 3));
+        // bricks per level < 2^21 (the layout's limit): a 24-bit multiply (full
+        // rate; the 3-plane group's stride 192 is not a power of two)
+        return __umul24((unsigned)nb, (unsigned)brick_stride(i)) + (unsigned)((px & 3) + 4 * (py & 3) + 16 * (pz & 3));
     };
     R f[Q];
     if (sizeof(T) == 2 && simple) {
@@ -106,22 +108,24 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
     bgk(f, inv_tau_of<R>(L, inst));
     if constexpr (std::is_same<T, _Float16>::value || std::is_same<T, fp16c>::value) {
         // rounding bits from the cell's own coordinates, level and step, so an
-        // instance rounds the same way alone and inside any batch
+        // instance rounds the same way alone and inside any batch.  The
+        // brick's coordinates, level and step are hashed once per wave (all
+        // wave-uniform: scalar-unit multiplies), then one hash of that plus
+        // the lane gives the cell's h.  Direction i draws h + i * 0x9E3779B9
+        // (a Weyl sequence: one add per direction, where the kernel is bound
+        // by VALU issue in 16-bit storage).  Each draw is as uniform over
+        // cells and steps as h, so each direction's rounding stays unbiased;
+        // the 19 draws of one cell are spread evenly over the 32-bit range.
         const int32_t* c3 = L.bcoord + 3 * b;
-        const uint32_t gx = (uint32_t)(c3[0] * 4 + lx), gy = (uint32_t)(c3[1] * 4 + ly),
-                       gz = (uint32_t)(c3[2] * 4 + lz);
-        // one hash per cell, then a xorshift32 stream for the 19 directions
-        // (shifts and xors only: the kernel is bound by VALU issue in 16-bit
-        // storage, and a per-direction hash costs two quarter-rate multiplies)
-        uint32_t x = mix32(gx * 0x9E3779B1u ^ mix32(gy * 0x85EBCA77u ^ mix32(gz * 0xC2B2AE3Du ^
-                               ((uint32_t)L.level << 24) ^ (*L.tick * 0x27D4EB2Fu)))) | 1u;
+        const uint32_t hb = mix32((uint32_t)c3[0] * 0x9E3779B1u ^
+                                  mix32((uint32_t)c3[1] * 0x85EBCA77u ^
+                                        mix32((uint32_t)c3[2] * 0xC2B2AE3Du ^ ((uint32_t)L.level << 24) ^
+                                              (*L.tick * 0x27D4EB2Fu))));
+        const uint32_t h = mix32(hb + (uint32_t)lane);
 #pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            x ^= x << 13;
-            x ^= x >> 17;
-            x ^= x << 5;
-            Store<T>::store_sr(B + plane_base(i, S), (unsigned)b * (unsigned)brick_stride(i) + (unsigned)lane, i, f[i], x);
-        }
+        for (int i = 0; i < Q; ++i)
+            Store<T>::store_sr(B + plane_base(i, S), (unsigned)b * (unsigned)brick_stride(i) + (unsigned)lane, i, f[i],
+                               h + (uint32_t)i * 0x9E3779B9u);
     } else {
 #pragma unroll
         for (int i = 0; i < Q; ++i)
