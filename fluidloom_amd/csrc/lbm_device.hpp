@@ -300,13 +300,43 @@ __device__ inline R special_link(const LevelDev& L, const T* __restrict__ A, int
     return Store<T>::load(A, pidx(i, L.S, (size_t)s2), i);
 }
 
+// Offset of the source that cell (lx, ly, lz) of a brick pulls in direction
+// i, from plane i's base (A + plane_base(i, S)).  NB: the brick's 27
+// neighbour entries (13 = the brick itself); the source brick is one of at
+// most four of them, chosen by whether the pulled position leaves the brick
+// along each axis (static indices only, so NB stays in registers).  Bricks
+// per level are below 2^21 (the layout's limit): a 24-bit multiply (full
+// rate; the 3-plane group's stride 192 is not a power of two).
+__device__ __attribute__((always_inline)) inline unsigned pull_offset(const int (&NB)[27], int lx, int ly, int lz,
+                                                                      int i) {
+    const int px = lx - cx(i), py = ly - cy(i), pz = lz - cz(i);
+    const bool ex = (unsigned)px > 3u, ey = (unsigned)py > 3u, ez = (unsigned)pz > 3u;
+    const int kx = -cx(i), ky = -3 * cy(i), kz = -9 * cz(i);
+    const int n000 = NB[13], n100 = NB[13 + kx], n010 = NB[13 + ky], n110 = NB[13 + kx + ky];
+    const int n001 = NB[13 + kz], n101 = NB[13 + kx + kz], n011 = NB[13 + ky + kz], n111 = NB[13 + kx + ky + kz];
+    const int nb = ez ? (ey ? (ex ? n111 : n011) : (ex ? n101 : n001)) : (ey ? (ex ? n110 : n010) : (ex ? n100 : n000));
+    return __umul24((unsigned)nb, (unsigned)brick_stride(i)) + (unsigned)((px & 3) + 4 * (py & 3) + 16 * (pz & 3));
+}
+
 // Generic pull of the 19 incoming populations of one slot (list kernels).
+// w: the slot's link word (L.cflags[slot]), which callers may load early.
 template <typename T, typename R>
-__device__ inline void gather_slot(const LevelDev& L, const T* __restrict__ A, int64_t slot, R* f) {
+__device__ inline void gather_slot(const LevelDev& L, const T* __restrict__ A, int64_t slot, uint64_t w, R* f) {
     const int b = (int)(slot >> 6);
     const int l = (int)(slot & 63);
     const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
-    const uint64_t w = L.cflags[slot];
+    if ((w & ((1ull << (2 * Q)) - 1)) == 0) {
+        // every link regular (the common case, e.g. the children at a level
+        // interface): the brick's neighbour row first, then 19 independent
+        // population loads (a regular link's source brick exists)
+        const int* __restrict__ row = L.nbr + (size_t)b * 27;
+        int NB[27];
+#pragma unroll
+        for (int k = 0; k < 27; ++k) NB[k] = row[k];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f[i] = Store<T>::load(A + plane_base(i, L.S), pull_offset(NB, lx, ly, lz, i), i);
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         const unsigned code = (unsigned)(w >> (2 * i)) & 3u;
@@ -316,6 +346,11 @@ __device__ inline void gather_slot(const LevelDev& L, const T* __restrict__ A, i
         else
             f[i] = special_link<T, R>(L, A, b, (size_t)slot, i, code, px, py, pz);
     }
+}
+
+template <typename T, typename R>
+__device__ inline void gather_slot(const LevelDev& L, const T* __restrict__ A, int64_t slot, R* f) {
+    gather_slot<T, R>(L, A, slot, L.cflags[slot], f);
 }
 
 }  // namespace lbm

@@ -72,18 +72,7 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
     const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
     const size_t S = L.S;
     // pulled source of direction i: offset from plane i's base (A + plane_base)
-    auto source = [&](int i) -> unsigned {
-        const int px = lx - cx(i), py = ly - cy(i), pz = lz - cz(i);
-        const bool ex = (unsigned)px > 3u, ey = (unsigned)py > 3u, ez = (unsigned)pz > 3u;
-        const int kx = -cx(i), ky = -3 * cy(i), kz = -9 * cz(i);
-        const int n000 = NB[13], n100 = NB[13 + kx], n010 = NB[13 + ky], n110 = NB[13 + kx + ky];
-        const int n001 = NB[13 + kz], n101 = NB[13 + kx + kz], n011 = NB[13 + ky + kz], n111 = NB[13 + kx + ky + kz];
-        const int nb = ez ? (ey ? (ex ? n111 : n011) : (ex ? n101 : n001))
-                          : (ey ? (ex ? n110 : n010) : (ex ? n100 : n000));
-        // bricks per level < 2^21 (the layout's limit): a 24-bit multiply (full
-        // rate; the 3-plane group's stride 192 is not a power of two)
-        return __umul24((unsigned)nb, (unsigned)brick_stride(i)) + (unsigned)((px & 3) + 4 * (py & 3) + 16 * (pz & 3));
-    };
+    auto source = [&](int i) -> unsigned { return pull_offset(NB, lx, ly, lz, i); };
     R f[Q];
     if (sizeof(T) == 2 && simple) {
         // 16-bit storage, every cell fluid, every link regular: 19 independent
@@ -206,12 +195,20 @@ __global__ __launch_bounds__(kWG) void k_fill_c2f(LevelDev Lf, const uint32_t* _
     const bool on = k < ngroups;
     uint32_t ci[kC2FCaps];
     uint32_t g = kNoCap, want = 0;
+    bool live = false;
+    double tau0 = 0;
     R pv[3] = {0, 0, 0}, mv[3] = {0, 0, 0};
     if (on) {
 #pragma unroll
         for (int j = 0; j < kC2FCaps; ++j) ci[j] = gcap[(size_t)kC2FCaps * k + j];
         g = gchild[(size_t)8 * k + q];
         want = gmask[(size_t)8 * k + q];
+        // the ghost's instance, looked up while the capture loads are in flight
+        if (g != kNoCap) {
+            const InstDev& I = Lf.inst[Lf.binst[g >> 6]];
+            live = I.live != 0;
+            tau0 = I.tau0;
+        }
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             const int i = q + 8 * r;  // population i of P
@@ -230,10 +227,8 @@ __global__ __launch_bounds__(kWG) void k_fill_c2f(LevelDev Lf, const uint32_t* _
     for (int i = 0; i < Q; ++i) f[i] = __shfl(pv[i >> 3], i & 7, 8);
 #pragma unroll
     for (int j = 0; j < 24; ++j) nm[j] = __shfl(mv[j >> 3], j & 7, 8);
-    if (g == kNoCap) return;
-    const InstDev& I = Lf.inst[Lf.binst[g >> 6]];
-    if (!I.live) return;
-    const R fac = (R)c2f_factor(I.tau0, Lf.level);
+    if (!live) return;
+    const R fac = (R)c2f_factor(tau0, Lf.level);
     R mp[4];
     moments(f, mp[0], mp[1], mp[2], mp[3]);
     R phi[4] = {mp[0], mp[1], mp[2], mp[3]};
@@ -306,9 +301,11 @@ __device__ inline void child_average(const LevelDev& Lf, const T* __restrict__ A
 }
 
 // Fine -> coarse: eight lanes per ghost, lane q pulling child q; the group
-// sums density, momentum and non-equilibrium parts with xor shuffles (a
+// sums density, momentum and non-equilibrium parts with DPP lane swaps (a
 // fixed tree, so the result does not depend on the launch) and each lane
-// then stores up to three of the ghost's 19 populations.
+// then stores up to three of the ghost's 19 populations.  The child's slot
+// and link word are loaded alongside the ghost's instance check, so the
+// dependent loads before the population reads are two deep, not four.
 template <typename T>
 __global__ __launch_bounds__(kWG) void k_fill_f2c(LevelDev Lc, const uint32_t* __restrict__ ghost,
                                                   const uint32_t* __restrict__ child8, uint32_t n, LevelDev Lf,
@@ -320,8 +317,12 @@ __global__ __launch_bounds__(kWG) void k_fill_f2c(LevelDev Lc, const uint32_t* _
     // every lane of a group shares k, so a group is wholly in or out
     bool on = k < n;
     size_t g = 0;
+    uint32_t child = 0;
+    uint64_t w = 0;
     if (on) {
         g = ghost[k];
+        child = child8[(size_t)8 * k + q];
+        w = Lf.cflags[child];
         on = Lc.inst[Lc.binst[g >> 6]].live != 0;
     }
     R s[4 + Q];
@@ -329,7 +330,7 @@ __global__ __launch_bounds__(kWG) void k_fill_f2c(LevelDev Lc, const uint32_t* _
     for (int j = 0; j < 4 + Q; ++j) s[j] = 0;
     if (on) {
         R f[Q];
-        gather_slot<T, R>(Lf, Af, child8[(size_t)8 * k + q], f);
+        gather_slot<T, R>(Lf, Af, child, w, f);
         R rho, vx, vy, vz;
         moments(f, rho, vx, vy, vz);
         const R usq = vx * vx + vy * vy + vz * vz;
@@ -340,10 +341,9 @@ __global__ __launch_bounds__(kWG) void k_fill_f2c(LevelDev Lc, const uint32_t* _
 #pragma unroll
         for (int i = 0; i < Q; ++i) s[4 + i] = f[i] - feq(i, rho, vx, vy, vz, usq);
     }
+    // the grid has whole waves and no lane has left: every lane is active
 #pragma unroll
-    for (int m = 1; m < 8; m <<= 1)
-#pragma unroll
-        for (int j = 0; j < 4 + Q; ++j) s[j] += __shfl_xor(s[j], m, 8);
+    for (int j = 0; j < 4 + Q; ++j) s[j] = group8_sum(s[j]);
     if (!on) return;
     const R fac = (R)f2c_factor(Lc.inst[Lc.binst[g >> 6]].tau0, Lc.level);
     const R r = s[0] * (R)0.125;
