@@ -20,7 +20,15 @@ A request's trajectory does not depend on what it is batched with (bricks of
 different instances never interact, reductions are per instance), so its
 result and its step count are the same alone and in the mix.
 
+The plain server is the comparison: the same trace served one request at a
+time (one cohort of one request, FP32 storage, eager steps), i.e. what a
+single-instance solver loop does.  Requests still unserved when a run stops
+at its wall limit enter the latency percentiles with the time they had
+waited so far (a lower bound), so an overloaded server is not flattered by
+counting only what it finished.
+
     python -m fluidloom_amd.serving --rate 20 --duration 600
+    python -m fluidloom_amd.serving --rate 4 --duration 60 --scale 2 --mode both
 """
 from __future__ import annotations
 
@@ -125,9 +133,10 @@ class Cohort:
 
 class Server:
     def __init__(self, storage="f16", check_every=100, tol=1e-6, max_steps=20000, mem_fraction=0.9,
-                 max_cohort=64, graph=True, keep_results=False):
+                 max_cohort=64, graph=True, keep_results=False, max_cohorts=None):
         self.storage, self.K, self.tol, self.max_steps = storage, check_every, tol, max_steps
         self.max_cohort, self.graph, self.keep_results = max_cohort, graph, keep_results
+        self.max_cohorts = max_cohorts  # None: as many concurrent cohorts as memory admits
         free, total = C_mem_info()
         self.budget = int(min(free, HBM_CAP) * mem_fraction)
         self.used = 0
@@ -135,6 +144,8 @@ class Server:
         self.done: list[Request] = []
 
     def _admit(self, waiting, now):
+        if self.max_cohorts is not None and len(self.cohorts) >= self.max_cohorts:
+            return 0
         batch = []
         while waiting and waiting[0].arrival <= now and len(batch) < self.max_cohort:
             r = waiting[0]
@@ -207,10 +218,15 @@ class Server:
             if ticks % 20 == 0:
                 log(f"t={time.perf_counter() - t0:.1f}s done={len(self.done)} waiting={len(waiting)} "
                     f"cohorts={len(self.cohorts)} used={self.used / 2**30:.2f} GiB")
+        t_end = time.perf_counter()
         for r in self.done:
             r.arrival -= t0
             r.admitted -= t0
             r.finished -= t0
+        # arrived but unfinished: how long each has waited so far
+        pending = [r for r in waiting if r.arrival <= t_end]
+        pending += [r for c in self.cohorts for k, r in enumerate(c.reqs) if c.live[k]]
+        self.pending_wait = [t_end - r.arrival for r in pending]
         self.unserved = len(waiting) + sum(sum(c.live) for c in self.cohorts)
         return self.done
 
@@ -222,13 +238,31 @@ def C_mem_info():
     return int(free.value), int(total.value)
 
 
-def summary(done, wall, unserved):
-    lat = np.array([r.finished - r.arrival for r in done]) if done else np.zeros(1)
+def summary(done, wall, unserved, pending_wait=()):
+    """Latency percentiles over every request that arrived: finished ones
+    with their time to solution, unfinished ones with their wait so far
+    (counted in `censored`; the percentiles are then lower bounds)."""
+    lat = [r.finished - r.arrival for r in done] + list(pending_wait)
+    lat = np.array(lat) if lat else np.zeros(1)
     upd = sum(r.updates_per_step * r.steps for r in done)
-    return dict(served=len(done), unserved=unserved, converged=sum(r.converged for r in done),
+    return dict(served=len(done), unserved=unserved, censored=len(pending_wait),
+                converged=sum(r.converged for r in done),
                 p50_s=round(float(np.percentile(lat, 50)), 3), p99_s=round(float(np.percentile(lat, 99)), 3),
                 mean_queue_s=round(float(np.mean([r.admitted - r.arrival for r in done])) if done else 0.0, 3),
-                mlups=round(upd / wall / 1e6, 1), wall_s=round(wall, 2))
+                mlups=round(upd / wall / 1e6, 1), solves_per_s=round(len(done) / wall, 3), wall_s=round(wall, 2))
+
+
+def plain_server(check_every=100, tol=1e-6, max_steps=20000):
+    """One request at a time, FP32, eager: the single-instance baseline."""
+    return Server("f32", check_every, tol, max_steps, max_cohort=1, graph=False, max_cohorts=1)
+
+
+def serve(srv, reqs, wall_limit):
+    t0 = time.perf_counter()
+    done = srv.run(reqs, wall_limit, t0)
+    wall = time.perf_counter() - t0
+    return dict(storage=srv.storage, budget_gib=round(srv.budget / 2**30, 1),
+                **summary(done, wall, srv.unserved, srv.pending_wait))
 
 
 def main():
@@ -241,19 +275,30 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--storage", default="f16")
     ap.add_argument("--wall-limit", type=float, default=None)
+    ap.add_argument("--mode", default="batched", choices=["batched", "plain", "both"])
+    ap.add_argument("--plain-wall-limit", type=float, default=None,
+                    help="wall limit of the plain run (default: twice the trace duration)")
     ap.add_argument("--seed", type=int, default=42)
     a = ap.parse_args()
     check(lib().fl_init(0))
-    reqs = request_specs(a.rate, a.duration, a.seed, a.scale)
-    log(f"{len(reqs)} requests over {a.duration:g} s")
-    srv = Server(a.storage, a.check_every, a.tol, a.max_steps)
-    t0 = time.perf_counter()
-    done = srv.run(reqs, a.wall_limit, t0)
-    wall = time.perf_counter() - t0
     out = dict(workload=f"Poisson {a.rate:g} req/s for {a.duration:g} s, 70% cavities / 30% spheres, "
-                        f"sizes / {a.scale}, steady state |du|/step < {a.tol:g} or {a.max_steps} steps, "
-                        f"{a.storage} storage", requests=len(reqs), budget_gib=round(srv.budget / 2**30, 1),
-               **summary(done, wall, srv.unserved))
+                        f"sizes / {a.scale}, steady state |du|/step < {a.tol:g} or {a.max_steps} steps")
+    if a.mode in ("batched", "both"):
+        reqs = request_specs(a.rate, a.duration, a.seed, a.scale)
+        log(f"continuous batching: {len(reqs)} requests over {a.duration:g} s")
+        out["requests"] = len(reqs)
+        out["batched"] = serve(Server(a.storage, a.check_every, a.tol, a.max_steps), reqs, a.wall_limit)
+    if a.mode in ("plain", "both"):
+        reqs = request_specs(a.rate, a.duration, a.seed, a.scale)
+        limit = a.plain_wall_limit if a.plain_wall_limit is not None else 2 * a.duration
+        log(f"plain (one request at a time, FP32, eager): {len(reqs)} requests, wall limit {limit:g} s")
+        out["requests"] = len(reqs)
+        out["plain"] = serve(plain_server(a.check_every, a.tol, a.max_steps), reqs, limit)
+    if a.mode == "both":
+        b, p = out["batched"], out["plain"]
+        out["p99_plain_over_batched"] = round(p["p99_s"] / max(b["p99_s"], 1e-9), 2)
+        out["p50_plain_over_batched"] = round(p["p50_s"] / max(b["p50_s"], 1e-9), 2)
+        out["solves_per_s_batched_over_plain"] = round(b["solves_per_s"] / max(p["solves_per_s"], 1e-9), 2)
     print(json.dumps(out), flush=True)
 
 

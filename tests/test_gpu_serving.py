@@ -64,3 +64,32 @@ def test_steady_state_retires(fl):
     srv = Server(storage="f32", check_every=10, tol=1e-6, max_steps=1000)
     (d,) = srv.run([r])
     assert d.converged and d.steps == 10
+
+
+def test_plain_server_one_at_a_time_same_results(fl):
+    """The plain baseline serves one request at a time in arrival order, and
+    (FP32 both ways) gives each request the populations and step count that
+    continuous batching gives it."""
+    from fluidloom_amd.serving import Server, plain_server
+    plain = plain_server(check_every=20, tol=1e-7, max_steps=120)
+    plain.keep_results = True
+    peak = []
+    orig = plain._admit
+
+    def admit(waiting, now):
+        n = orig(waiting, now)
+        peak.append(len(plain.cohorts))
+        return n
+
+    plain._admit = admit
+    done = plain.run(small_requests())
+    assert [d.rid for d in done] == [0, 1, 2, 3] and plain.unserved == 0 and not plain.pending_wait
+    assert max(peak) == 1
+    for a, b in zip(done, done[1:]):
+        assert b.admitted >= a.finished
+    mix = Server(storage="f32", check_every=20, tol=1e-7, max_steps=120, keep_results=True)
+    batched = {r.rid: r for r in mix.run(small_requests())}
+    for d in done:
+        assert d.steps == batched[d.rid].steps
+        assert np.array_equal(d.result["f"], batched[d.rid].result["f"]), d.rid
+
