@@ -205,6 +205,9 @@ def main():
         t = json.loads(tf.read_text())
         if t.get("variant") == finest_variant:
             traffic = int(t["traffic_bytes"])
+        elif finest_variant in t.get("per_variant_traffic_bytes", {}):
+            # the same counter runs cover every variant (the sweep below runs them all)
+            traffic = int(t["per_variant_traffic_bytes"][finest_variant])
 
     # bandwidth reference: a plain 16 B/lane copy kernel over 2 x 2 GiB
     copy_gbs = copy_bandwidth(_lib, L, stream)

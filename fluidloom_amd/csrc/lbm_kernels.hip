@@ -40,6 +40,18 @@ __device__ inline unsigned xcd_remap(unsigned bid, unsigned nblk) {
     return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
+// XCD-aware in chunks of C workgroups: the k-th workgroup dispatched to XCD x
+// (workgroup b goes to XCD b % 8) takes item k % C of chunk (k / C) * 8 + x.
+// Each XCD then works through runs of C consecutive groups (a brick's
+// neighbours mostly in its own L2) while the eight XCDs stay inside one
+// window of 8C groups.  Groups past the last whole window keep their index.
+__device__ inline unsigned xcd_chunk_remap(unsigned bid, unsigned nblk, unsigned C) {
+    const unsigned win = 8u * C;
+    if (bid >= (nblk / win) * win) return bid;
+    const unsigned x = bid % 8u, k = bid / 8u;
+    return ((k / C) * 8u + x) * C + k % C;
+}
+
 template <typename R>
 __device__ inline R inv_tau_of(const LevelDev& L, int inst) {
     if (sizeof(R) == 8) return (R)L.itau_d[inst];
@@ -122,9 +134,14 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
     }
 }
 
-// MODE 0: workgroup g takes brick group g; 1: the same with the XCD-aware
-// remap; 2: persistent -- a grid sized to the resident capacity, each
-// workgroup walking a contiguous chunk of groups (chunks of one XCD adjacent).
+// MODE 0: workgroup g takes brick group g (consecutive groups land on
+// different XCDs); 1: XCD-aware remap, each XCD one contiguous eighth of the
+// groups; 2: persistent -- a grid sized to the resident capacity, each
+// workgroup walking a contiguous chunk of groups (chunks of one XCD
+// adjacent); 3 / 4: XCD-aware in chunks of 64 / 128 bricks (xcd_chunk_remap).
+// On the 3-level sphere's finest level (980k bricks) the chunked forms are
+// the fastest: mode 1 keeps neighbours in one L2 but streams eight distant
+// windows, mode 0 streams one window but splits neighbours over all L2s.
 template <typename T, int BPW, int MODE>
 __global__ __launch_bounds__(64 * BPW) void k_collide(LevelDev L, const T* __restrict__ A, T* __restrict__ B,
                                                       unsigned ngroups) {
@@ -136,7 +153,9 @@ __global__ __launch_bounds__(64 * BPW) void k_collide(LevelDev L, const T* __res
         const unsigned g1 = g0 + per < ngroups ? g0 + per : ngroups;
         for (unsigned g = g0; g < g1; ++g) collide_brick<T>(L, A, B, (int)(g * BPW) + wave);
     } else {
-        const unsigned g = MODE == 1 ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+        const unsigned g = MODE == 1   ? xcd_remap(blockIdx.x, gridDim.x)
+                           : MODE >= 3 ? xcd_chunk_remap(blockIdx.x, gridDim.x, (MODE == 3 ? 64u : 128u) / BPW)
+                                       : blockIdx.x;
         collide_brick<T>(L, A, B, (int)(g * BPW) + wave);
     }
 }
@@ -740,13 +759,13 @@ hipError_t collide_variant(const LevelDev& L, const T* A, T* B, hipStream_t s) {
 
 }  // namespace
 
-int collide_variants() { return 13; }
+int collide_variants() { return 17; }
 
 const char* collide_variant_name(int v) {
     static const char* names[] = {"bpw1",     "bpw2",         "bpw4",         "bpw1-xcd",     "bpw2-xcd", "bpw4-xcd",
                                   "bpw1-persist", "bpw2-persist", "bpw4-persist", "bpw8",     "bpw8-xcd", "bpw16",
-                                  "bpw16-xcd"};
-    return v >= 0 && v < 13 ? names[v] : "?";
+                                  "bpw16-xcd", "bpw2-xc128", "bpw4-xc64", "bpw4-xc128", "bpw8-xc128"};
+    return v >= 0 && v < 17 ? names[v] : "?";
 }
 
 template <typename T>
@@ -765,7 +784,11 @@ hipError_t launch_collide(const LevelDev& L, const T* A, T* B, int variant, hipS
         case 9: return collide_variant<T, 8, 0>(L, A, B, s);
         case 10: return collide_variant<T, 8, 1>(L, A, B, s);
         case 11: return collide_variant<T, 16, 0>(L, A, B, s);
-        default: return collide_variant<T, 16, 1>(L, A, B, s);
+        case 12: return collide_variant<T, 16, 1>(L, A, B, s);
+        case 13: return collide_variant<T, 2, 4>(L, A, B, s);
+        case 14: return collide_variant<T, 4, 3>(L, A, B, s);
+        case 15: return collide_variant<T, 4, 4>(L, A, B, s);
+        default: return collide_variant<T, 8, 4>(L, A, B, s);
     }
 }
 
