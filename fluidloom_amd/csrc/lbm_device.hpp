@@ -40,15 +40,15 @@ constexpr bool pairs_adjacent() {
 }
 static_assert(pairs_adjacent(), "lattice declaration must list opposite velocities in adjacent pairs");
 
-// Plane groups.  Planes 8g .. 8g+7 form group g (the last group holds the
+// Plane groups.  Planes 16g .. 16g+15 form group g (the last group holds the
 // remaining Q - 16 = 3), and inside a group each brick's 64-cell planes sit
-// next to each other: group g starts at element 8g * S, and brick b's run in
-// it at 64 * (group size) * b.  One brick's reads and writes are then three
-// contiguous runs per buffer (2 KiB, 2 KiB and 768 B in FP32) instead of 19
-// runs of 256 B, which HBM serves with fewer row activations.  (All 19
-// planes of a brick in one run was far slower: neighbouring waves then
-// crowd into a narrow address window.)
-constexpr int kPlaneGroup = 8;
+// next to each other: group g starts at element 16g * S, and brick b's run
+// in it at 64 * (group size) * b.  One brick's reads and writes are then two
+// contiguous runs per buffer (4 KiB and 768 B in FP32) instead of 19 runs of
+// 256 B, which HBM serves with fewer row activations.  (All 19 planes of a
+// brick in one run was far slower: neighbouring waves then crowd into a
+// narrow address window.  Groups of 8 measured 1-3 % slower.)
+constexpr int kPlaneGroup = 16;
 __host__ __device__ constexpr int plane_group_size(int i) {
     return (i / kPlaneGroup + 1) * kPlaneGroup <= Q ? kPlaneGroup : Q % kPlaneGroup;
 }
