@@ -70,6 +70,14 @@ __device__ inline uint32_t find_cell(const CellHash& h, int l, int32_t x, int32_
     return cell_lookup(h.tk, h.tv, h.cap, cell_hkey((uint32_t)x, (uint32_t)y, (uint32_t)z, l));
 }
 
+// The key find_cell looks up, or EMPTY (absent) outside the key's range.
+__device__ inline uint64_t cell_key_or_empty(int l, int32_t x, int32_t y, int32_t z) {
+    if (l < 0 || l > 19 || x < 0 || y < 0 || z < 0 || x >= kKeyCoordLimit || y >= kKeyCoordLimit ||
+        z >= kKeyCoordLimit)
+        return kEmptyKey;
+    return cell_hkey((uint32_t)x, (uint32_t)y, (uint32_t)z, l);
+}
+
 struct Root {
     int32_t n[3];
 };
@@ -113,31 +121,56 @@ fl_status build_cell_hash(CellsC c, size_t n, uint64_t* keys, uint64_t* tk, uint
 }
 
 // ---------------------------------------------------------------- a9
+// Per face: the first cell holding the face point at levels L, L-1, .., 0;
+// else kFiner when a level L+1 cell touches it; else kInvalid.  The six
+// faces are searched in lockstep, level by level (cell_lookup_n), so a
+// cell's lookups overlap; each face's result is the same first hit.
 __global__ __launch_bounds__(kBlock) void k_face_nbr(CellsC c, size_t n, const Root root, CellHash h,
                                                      uint32_t* __restrict__ nbr) {
     for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
         const int L = c.level[i];
+        const int32_t x = c.x[i], y = c.y[i], z = c.z[i];
+        int32_t px[6], py[6], pz[6];
+        uint32_t r[6];
+        unsigned pend = 0;
+#pragma unroll
         for (int f = 0; f < 6; ++f) {
-            int32_t px = c.x[i] + kF6[f][0], py = c.y[i] + kF6[f][1], pz = c.z[i] + kF6[f][2];
-            uint32_t r = kInvalid;
-            if (in_domain(L, root, px, py, pz)) {
-                for (int l = L; l >= 0; --l) {
-                    int sh = L - l;
-                    uint32_t j = find_cell(h, l, px >> sh, py >> sh, pz >> sh);
-                    if (j != kInvalid) { r = j; break; }
-                }
-                if (r == kInvalid) {
-                    for (int q = 0; q < 8; ++q) {
-                        if (find_cell(h, L + 1, 2 * px + (q & 1), 2 * py + ((q >> 1) & 1),
-                                      2 * pz + ((q >> 2) & 1)) != kInvalid) {
-                            r = kFiner;
-                            break;
-                        }
-                    }
-                }
-            }
-            nbr[6 * i + f] = r;
+            px[f] = x + kF6[f][0];
+            py[f] = y + kF6[f][1];
+            pz[f] = z + kF6[f][2];
+            r[f] = kInvalid;
+            if (in_domain(L, root, px[f], py[f], pz[f])) pend |= 1u << f;
         }
+        for (int l = L; l >= 0 && pend; --l) {
+            const int sh = L - l;
+            uint64_t key[6];
+            uint32_t got[6];
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+                key[f] = ((pend >> f) & 1u) ? cell_key_or_empty(l, px[f] >> sh, py[f] >> sh, pz[f] >> sh) : kEmptyKey;
+            cell_lookup_n<6>(h.tk, h.tv, h.cap, key, got);
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+                if (((pend >> f) & 1u) && got[f] != kInvalid) {
+                    r[f] = got[f];
+                    pend &= ~(1u << f);
+                }
+        }
+        for (int f = 0; f < 6; ++f) {
+            if (!((pend >> f) & 1u)) continue;
+            uint64_t key[8];
+            uint32_t got[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                key[q] = cell_key_or_empty(L + 1, 2 * px[f] + (q & 1), 2 * py[f] + ((q >> 1) & 1),
+                                           2 * pz[f] + ((q >> 2) & 1));
+            cell_lookup_n<8>(h.tk, h.tv, h.cap, key, got);
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (got[q] != kInvalid) r[f] = kFiner;
+        }
+#pragma unroll
+        for (int f = 0; f < 6; ++f) nbr[6 * i + f] = r[f];
     }
 }
 
@@ -178,22 +211,37 @@ __global__ __launch_bounds__(kBlock) void k_detect0(CellsC c, size_t n, int M,
             const int L = c.level[i];
             const int32_t ax = c.x[i] << (M - L), ay = c.y[i] << (M - L), az = c.z[i] << (M - L);
             const int32_t size = s <= M ? (1 << (M - s)) : 0;
-            for (int t = 0; t < 6 && !v; ++t) {
-                int32_t px = ax, py = ay, pz = az;
-                int axis = t >> 1;
-                int32_t d = (t & 1) ? size : -1;
-                if (axis == 0) px += d; else if (axis == 1) py += d; else pz += d;
-                for (int l = 0; l <= M; ++l) {
-                    int32_t msk = ~((1 << (M - l)) - 1);
-                    int32_t qx = px & msk, qy = py & msk, qz = pz & msk;
-                    if (qx < 0 || qy < 0 || qz < 0 || qx >= (1 << M) || qy >= (1 << M) || qz >= (1 << M))
-                        continue;
-                    uint64_t key = cell_hkey((uint32_t)qx, (uint32_t)qy, (uint32_t)qz, l);
-                    if (cell_lookup(h.tk, h.tv, h.cap, key) != kInvalid) {
-                        if (l > s + 1) v = 1;
-                        break;
-                    }
+            // per face point: the first level l = 0..M holding it decides; the
+            // six points are probed in lockstep (cell_lookup_n), which gives
+            // the same first hits, and the cell violates if any face does
+            int32_t px[6], py[6], pz[6];
+#pragma unroll
+            for (int t = 0; t < 6; ++t) {
+                px[t] = ax;
+                py[t] = ay;
+                pz[t] = az;
+                const int32_t d = (t & 1) ? size : -1;
+                if ((t >> 1) == 0) px[t] += d; else if ((t >> 1) == 1) py[t] += d; else pz[t] += d;
+            }
+            unsigned pend = 0x3Fu;
+            for (int l = 0; l <= M && pend && !v; ++l) {
+                const int32_t msk = ~((1 << (M - l)) - 1);
+                uint64_t key[6];
+                uint32_t got[6];
+#pragma unroll
+                for (int t = 0; t < 6; ++t) {
+                    const int32_t qx = px[t] & msk, qy = py[t] & msk, qz = pz[t] & msk;
+                    const bool in = qx >= 0 && qy >= 0 && qz >= 0 && qx < (1 << M) && qy < (1 << M) && qz < (1 << M);
+                    key[t] = ((pend >> t) & 1u) && in ? cell_hkey((uint32_t)qx, (uint32_t)qy, (uint32_t)qz, l)
+                                                      : kEmptyKey;
                 }
+                cell_lookup_n<6>(h.tk, h.tv, h.cap, key, got);
+#pragma unroll
+                for (int t = 0; t < 6; ++t)
+                    if (((pend >> t) & 1u) && got[t] != kInvalid) {
+                        if (l > s + 1) v = 1;
+                        pend &= ~(1u << t);
+                    }
             }
         }
         viol[i] = v;

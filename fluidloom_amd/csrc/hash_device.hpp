@@ -111,4 +111,46 @@ __device__ inline uint32_t cell_lookup(const uint64_t* __restrict__ tkeys, const
     return 0xFFFFFFFFu;
 }
 
+// N lookups advanced together, one probe step each per round: their
+// dependent loads overlap instead of one chain waiting on another.  The
+// value is loaded beside the key so a hit costs no extra round.  out[j]:
+// the stored value, or 0xFFFFFFFF when key[j] is absent (or is EMPTY).
+template <int N>
+__device__ inline void cell_lookup_n(const uint64_t* __restrict__ tkeys, const uint32_t* __restrict__ tvals,
+                                     uint64_t cap, const uint64_t (&key)[N], uint32_t (&out)[N]) {
+    uint64_t s[N];
+    unsigned open = 0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        out[j] = 0xFFFFFFFFu;
+        s[j] = cell_slot(key[j], cap);
+        if (key[j] != 0xFFFFFFFFFFFFFFFFull) open |= 1u << j;
+    }
+    for (uint64_t p = 0; p < cap && open; ++p) {
+        uint64_t k[N];
+        uint32_t v[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            k[j] = 0xFFFFFFFFFFFFFFFFull;
+            v[j] = 0xFFFFFFFFu;
+            if ((open >> j) & 1u) {
+                k[j] = tkeys[s[j]];
+                v[j] = tvals[s[j]];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            if (!((open >> j) & 1u)) continue;
+            if (k[j] == key[j]) {
+                out[j] = v[j];
+                open &= ~(1u << j);
+            } else if (k[j] == 0xFFFFFFFFFFFFFFFFull) {
+                open &= ~(1u << j);
+            } else {
+                s[j] = (s[j] + 1) & (cap - 1);
+            }
+        }
+    }
+}
+
 }  // namespace fl
