@@ -50,6 +50,10 @@ __global__ void k_zero_u32(uint32_t* __restrict__ p, size_t n) {
 // Upfront digit histograms of all passes in one read of the keys.  Each
 // wave counts into its own LDS copy (fewer conflicting LDS atomics), each
 // thread takes kHistItems keys of a block tile with all loads issued first.
+// Lanes holding the same digit as the wave's first active lane are counted
+// with one ballot and added by one lane: a digit shared by most of the wave
+// (the zero low digits of coarse cells' keys, a constant high digit) would
+// otherwise serialise 64 atomics on one LDS word.
 constexpr int kHistItems = 8;
 
 __global__ __launch_bounds__(kThreads) void k_hist(const uint64_t* __restrict__ keys, size_t n,
@@ -74,7 +78,12 @@ __global__ __launch_bounds__(kThreads) void k_hist(const uint64_t* __restrict__ 
                 const int b = begin_bit + 8 * p;
                 const int nb = end_bit - b < 8 ? end_bit - b : 8;
                 const unsigned d = (unsigned)((k[j] >> b) & ((1ull << nb) - 1ull));
-                atomicAdd(&hw[p * 256 + d], 1u);
+                const unsigned d0 = __builtin_amdgcn_readfirstlane(d);
+                const uint64_t same = __ballot(d == d0);
+                if (d != d0)
+                    atomicAdd(&hw[p * 256 + d], 1u);
+                else if (lane_id() == (unsigned)__builtin_ctzll(same))
+                    atomicAdd(&hw[p * 256 + d0], (uint32_t)__builtin_popcountll(same));
             }
         }
     }
