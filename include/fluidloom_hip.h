@@ -358,9 +358,12 @@ fl_status fl_lbm_set_live(fl_lbm_solver* solver, int inst, int live);
  * (captured on first use, re-captured after a regrid or variant change). */
 fl_status fl_lbm_set_graph(fl_lbm_solver* solver, int enable);
 fl_status fl_lbm_stream(fl_lbm_solver* solver, fl_stream* s);
-/* Collide-stream variants (bricks per workgroup x launch shape) and the
- * autotuner that times each on the live layout and keeps the fastest per
- * level (h_chosen: one entry per level, may be NULL). */
+/* Collide-stream variants (bricks per workgroup x workgroup-to-brick
+ * mapping: in order, XCD-contiguous, XCD chunks of 64 / 128 bricks,
+ * persistent) and the autotuner: it times every variant on the live layout
+ * (`iters` launches each), re-times the three fastest three more times each
+ * and keeps the fastest per level (h_chosen: one entry per level, may be
+ * NULL).  All variants give the same bits. */
 int fl_lbm_num_variants(void);
 const char* fl_lbm_variant_name(int variant);
 fl_status fl_lbm_set_variant(fl_lbm_solver* solver, int level, int variant);
