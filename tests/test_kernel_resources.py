@@ -63,3 +63,23 @@ def test_collide_occupancy(usage):
     assert len(full) == 42
     short = {k: o for k, o in full.items() if o < 8}
     assert not short, short
+
+
+def test_device_code_keeps_fp32_denormals(tmp_path):
+    """The FP16C codec maps its codes below 2^-13 onto FP32 denormals
+    (fp16c_codec.hpp); a flush-to-zero build would silently zero them.  A
+    kernel using the codec, built with the library's flags, must declare
+    FP32 denormal mode 3 (inputs and outputs kept) in its descriptor."""
+    src = tmp_path / "denorm.hip"
+    src.write_text('#include "fp16c_codec.hpp"\n'
+                   "__global__ void k(const unsigned short* c, float* f, unsigned short* e) {\n"
+                   "    const unsigned i = threadIdx.x;\n"
+                   "    f[i] = fl::lbm::fp16c_decode(c[i]);\n"
+                   "    e[i] = fl::lbm::fp16c_encode_sr_scaled(f[i] * fl::lbm::kFp16cDown, i);\n"
+                   "}\n")
+    out = tmp_path / "denorm.s"
+    cmd = [build.HIPCC, *build.COMMON, "--offload-device-only", "-S", str(src), "-o", str(out)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    modes = re.findall(r"\.amdhsa_float_denorm_mode_32\s+(\d+)", out.read_text())
+    assert modes and all(m == "3" for m in modes), modes
