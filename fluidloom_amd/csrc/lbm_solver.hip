@@ -728,6 +728,21 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
                 H.f2c_child[8 * k + q] =
                     (uint32_t)F.slot_at(H.binst[b], 2 * x + (q & 1), 2 * y + ((q >> 1) & 1), 2 * z + ((q >> 2) & 1));
         }
+        // ghosts in the order of their first child's fine slot: the eight
+        // ghosts over one fine brick become adjacent, so a wave's gathers read
+        // one brick's planes (and their neighbours') instead of eight bricks'
+        const size_t n = H.f2c_ghost.size();
+        std::vector<uint32_t> ord(n);
+        for (size_t k = 0; k < n; ++k) ord[k] = (uint32_t)k;
+        std::stable_sort(ord.begin(), ord.end(),
+                         [&](uint32_t a, uint32_t b2) { return H.f2c_child[8 * (size_t)a] < H.f2c_child[8 * (size_t)b2]; });
+        std::vector<uint32_t> g2(n), c2(8 * n);
+        for (size_t k = 0; k < n; ++k) {
+            g2[k] = H.f2c_ghost[ord[k]];
+            for (int q = 0; q < 8; ++q) c2[8 * k + q] = H.f2c_child[8 * (size_t)ord[k] + q];
+        }
+        H.f2c_ghost.swap(g2);
+        H.f2c_child.swap(c2);
     }
     for (int l = 0; l < NL; ++l)
         if ((uint64_t)S->h[l].nb * 64 >= (1ull << 32) / 32)
