@@ -1,6 +1,7 @@
 """AMR integer micro-benchmark on the device: 64M seeded leaf cells at levels
 0-10 in a 2^10 root domain, split Bernoulli(0.05), merge Bernoulli(0.03) on
-complete sibling groups; Hilbert keys, 64-bit key sort, one regrid pass
+complete sibling groups; the reference's rot/dir keys, Hilbert keys, 64-bit
+key sort, a 64M-element exclusive scan, one regrid pass
 (reference balance rule to fixpoint, split, merge, scan compaction) and the
 hash face-neighbour rebuild.  Each stage is timed with HIP events; GB/s is
 computed from the algorithmic bytes of that stage.
@@ -46,8 +47,14 @@ def run(n=64 << 20, seed=42, p_split=0.05, p_merge=0.03, max_level=10, log=None)
     tb = L.fl_regrid_temp_bytes(n)
     tmp = DeviceArray(tb, np.uint8)
     check(L.fl_gen_flags(C.byref(cs), n, seed, p_split, p_merge, tmp.p, tb, None))
-    # keygen: key63 | level (21 B/cell: 12 coords + 1 level + 8 key)
     keys = DeviceArray(n, np.uint64)
+    # rot/dir codec, the reference's merge / balance hash key at max_level 8
+    # (20 B/cell: 12 coords in, 8 key out)
+    with t:
+        check(L.fl_rotdir_encode(cells.x.p, cells.y.p, cells.z.p, 8, keys.p, n, None))
+    out["rotdir_ms"] = t.ms
+    out["rotdir_gbs"] = 20 * n / t.ms / 1e6
+    # keygen: key63 | level (21 B/cell: 12 coords + 1 level + 8 key)
     with t:
         check(L.fl_hilbert_key63(cells.x.p, cells.y.p, cells.z.p, cells.level.p, 1, keys.p, n, None))
     out["keygen_ms"] = t.ms
@@ -66,6 +73,16 @@ def run(n=64 << 20, seed=42, p_split=0.05, p_merge=0.03, max_level=10, log=None)
     # per pass) plus the upfront histogram read of the keys (8 B)
     out["sort_gbs"] = (8 + 8 * 24) * n / t.ms / 1e6
     del ka, va, st
+    # exclusive scan of n u32 (4 B read + 4 B written per element)
+    sin, sout, stot = DeviceArray(n, np.uint32), DeviceArray(n, np.uint32), DeviceArray(1, np.uint32)
+    check(L.fl_iota_u32(sin.p, n, None))
+    sctb = int(L.fl_scan_temp_bytes(n))
+    sct = DeviceArray(max(sctb, 1), np.uint8)
+    with t:
+        check(L.fl_scan_exclusive_u32(sin.p, sout.p, stot.p, n, sct.p, sctb, None))
+    out["scan_ms"] = t.ms
+    out["scan_gbs"] = 8 * n / t.ms / 1e6
+    del sin, sout, stot, sct
     # regrid pass
     cap = n + n // 2
     new = amr.GpuCells(cap)

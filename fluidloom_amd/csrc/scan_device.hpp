@@ -47,7 +47,7 @@ __device__ inline uint64_t wave_sum64(uint64_t v) {
 // it leaves in sh.prefix.  When `total` is non-null the last tile writes the
 // grand total there.  Called by wave 0 only.
 __device__ inline void tile_lookback(ScanShared& sh, uint64_t* ws, uint32_t tile, uint64_t tile_total,
-                                     uint64_t* total, size_t n) {
+                                     uint64_t* total, size_t n, size_t tile_elems = kScanTile) {
     const unsigned lane = lane_id();
     uint64_t* st = ws + 1;
     uint64_t excl = 0;
@@ -74,7 +74,7 @@ __device__ inline void tile_lookback(ScanShared& sh, uint64_t* ws, uint32_t tile
     }
     if (lane == 0) {
         sh.prefix = excl;
-        if (total && (size_t)(tile + 1) * kScanTile >= n) *total = excl + tile_total;
+        if (total && (size_t)(tile + 1) * tile_elems >= n) *total = excl + tile_total;
     }
 }
 
@@ -82,7 +82,7 @@ __device__ inline void tile_lookback(ScanShared& sh, uint64_t* ws, uint32_t tile
 // element; `mine` is the thread's total.  When `total` is non-null the last
 // tile writes the grand total there.  Contains __syncthreads().
 __device__ inline uint64_t scan_tile_prefix(ScanShared& sh, uint64_t* ws, uint32_t tile, uint64_t mine,
-                                            uint64_t* total, size_t n) {
+                                            uint64_t* total, size_t n, size_t tile_elems = kScanTile) {
     const int w = threadIdx.x >> 6;
     const unsigned lane = lane_id();
     uint64_t inc = wave_inclusive_scan64(mine);
@@ -95,7 +95,7 @@ __device__ inline uint64_t scan_tile_prefix(ScanShared& sh, uint64_t* ws, uint32
         if (i < w) woff += s;
         tile_total += s;
     }
-    if (w == 0) tile_lookback(sh, ws, tile, tile_total, total, n);
+    if (w == 0) tile_lookback(sh, ws, tile, tile_total, total, n, tile_elems);
     __syncthreads();
     return sh.prefix + woff + inc - mine;
 }

@@ -107,6 +107,23 @@ def test_scan_matches_oracle(fl, oracle, n):
         assert np.array_equal(out.to_host()[:n], ref)
 
 
+@pytest.mark.parametrize("off_in,off_out", [(1, 0), (0, 3), (2, 2)])
+def test_scan_unaligned_pointers(fl, oracle, off_in, off_out):
+    # the C ABI takes any element pointer: runs that are not 16 B aligned
+    # take the element-wise path and must give the same result
+    n = 3 * 4096 + 77
+    a = np.random.default_rng(off_in + 7 * off_out).integers(0, 9, n).astype(np.uint32)
+    d = dev(fl, np.concatenate([np.zeros(off_in, np.uint32), a]))
+    out = fl.DeviceArray(n + off_out, np.uint32)
+    tot = fl.DeviceArray(1, np.uint64)
+    tb = fl.lib().fl_scan_temp_bytes(n)
+    t = fl.DeviceArray(max(tb, 8), np.uint8)
+    fl.check(fl.lib().fl_scan_exclusive_u32(d.p + 4 * off_in, out.p + 4 * off_out, tot.p, n, t.p, tb, None))
+    ref, rt = oracle.scan_exclusive(a)
+    assert int(tot.to_host()[0]) == rt
+    assert np.array_equal(out.to_host()[off_out:off_out + n], ref)
+
+
 @pytest.mark.parametrize("n", [1, 1000, 300001])
 def test_hash_matches_oracle(fl, oracle, n):
     rng = np.random.default_rng(n)
