@@ -274,11 +274,35 @@ __device__ bool cube_violates(const CellHash& h, const uint8_t* __restrict__ sha
     return false;
 }
 
+// The cell violates if any of its 18 in-domain neighbour cubes does.  The
+// common case, a cube that is itself a leaf at level L (no violation), is
+// probed first for all directions, six lookups in lockstep at a time; only
+// the directions it leaves open take the full test, whose first probe is
+// that same lookup, so the answer is unchanged.
 __device__ bool detect_mode1(const CellHash& h, const uint8_t* __restrict__ shadow, int L, int32_t x,
                              int32_t y, int32_t z, int s, const Root root) {
+    uint32_t open = 0;
+    for (int d0 = 0; d0 < 18; d0 += 6) {
+        uint64_t key[6];
+        uint32_t got[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int d = d0 + k;
+            const int32_t px = x + kD18[d][0], py = y + kD18[d][1], pz = z + kD18[d][2];
+            key[k] = kEmptyKey;
+            if (in_domain(L, root, px, py, pz)) {
+                open |= 1u << d;
+                key[k] = cell_key_or_empty(L, px, py, pz);
+            }
+        }
+        cell_lookup_n<6>(h.tk, h.tv, h.cap, key, got);
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            if (got[k] != kInvalid) open &= ~(1u << (d0 + k));
+    }
     for (int d = 0; d < 18; ++d) {
-        int32_t px = x + kD18[d][0], py = y + kD18[d][1], pz = z + kD18[d][2];
-        if (!in_domain(L, root, px, py, pz)) continue;
+        if (!((open >> d) & 1u)) continue;
+        const int32_t px = x + kD18[d][0], py = y + kD18[d][1], pz = z + kD18[d][2];
         if (cube_violates(h, shadow, L, px, py, pz, s)) return true;
     }
     return false;
