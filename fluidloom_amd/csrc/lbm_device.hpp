@@ -215,8 +215,12 @@ constexpr uint64_t kActiveBit = 1ull << 63;
 // c2f capture indices per ghost: the covering coarse cell, then its coarse
 // fluid neighbours along -x, +x, -y, +y, -z, +z (kNoCap when absent)
 constexpr int kC2FCaps = 7;
-// planes of a capture buffer: the Q populations, then density and momentum
-constexpr int kCapPlanes = Q + 4;
+// A capture buffer of ncap cells: Q population planes, then one record of
+// (density, momentum x, y, z) per cell starting at cap_moments(ncap), which
+// rounds up to a multiple of 4 elements so a record is one aligned 16 B (FP32)
+// or 32 B (FP64) access.  Size: cap_elems(ncap).
+__host__ __device__ constexpr size_t cap_moments(size_t ncap) { return ((size_t)Q * ncap + 3) & ~(size_t)3; }
+__host__ __device__ constexpr size_t cap_elems(size_t ncap) { return cap_moments(ncap) + 4 * ncap; }
 constexpr uint32_t kNoCap = 0xFFFFFFFFu;
 enum : unsigned { kLinkRegular = 0, kLinkSolid = 1, kLinkWall = 2, kLinkOutflow = 3 };
 

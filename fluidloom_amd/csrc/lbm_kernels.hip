@@ -175,8 +175,8 @@ __global__ __launch_bounds__(kWG) void k_capture(LevelDev L, const T* __restrict
         density_momentum(f, m[0], m[1], m[2], m[3]);
 #pragma unroll
         for (int i = 0; i < Q; ++i) cap[(size_t)i * ncap + k] = f[i];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) cap[(size_t)(Q + j) * ncap + k] = m[j];
+        typedef R R4 __attribute__((ext_vector_type(4)));
+        *reinterpret_cast<R4*>(cap + cap_moments(ncap) + 4 * (size_t)k) = R4{m[0], m[1], m[2], m[3]};
     }
 }
 
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kWG) void k_fill_c2f(LevelDev Lf, const uint32_t* _
     uint32_t g = kNoCap, want = 0;
     bool live = false;
     double tau0 = 0;
-    R pv[3] = {0, 0, 0}, mv[3] = {0, 0, 0};
+    R pv[3] = {0, 0, 0}, mv[4] = {0, 0, 0, 0};
     if (on) {
 #pragma unroll
         for (int j = 0; j < kC2FCaps; ++j) ci[j] = gcap[(size_t)kC2FCaps * k + j];
@@ -232,10 +232,19 @@ __global__ __launch_bounds__(kWG) void k_fill_c2f(LevelDev Lf, const uint32_t* _
         for (int r = 0; r < 3; ++r) {
             const int i = q + 8 * r;  // population i of P
             if (i < Q) pv[r] = ((R)1 - alpha) * cap0[(size_t)i * ncap + ci[0]] + alpha * cap1[(size_t)i * ncap + ci[0]];
-            const int j = q + 8 * r;  // moment j & 3 of neighbour j >> 2 (j < 24)
-            const uint32_t c = ci[1 + (j >> 2)];
-            const size_t m = (size_t)(Q + (j & 3)) * ncap + c;
-            if (c != kNoCap) mv[r] = ((R)1 - alpha) * cap0[m] + alpha * cap1[m];
+        }
+        // lane a < 6: the moment record of neighbour a, one access per capture
+        if (q < 6) {
+            const uint32_t c = ci[1 + q];
+            if (c != kNoCap) {
+                typedef R R4 __attribute__((ext_vector_type(4)));
+                const R4 a0 = *reinterpret_cast<const R4*>(cap0 + cap_moments(ncap) + 4 * (size_t)c);
+                const R4 a1 = *reinterpret_cast<const R4*>(cap1 + cap_moments(ncap) + 4 * (size_t)c);
+                mv[0] = ((R)1 - alpha) * a0.x + alpha * a1.x;
+                mv[1] = ((R)1 - alpha) * a0.y + alpha * a1.y;
+                mv[2] = ((R)1 - alpha) * a0.z + alpha * a1.z;
+                mv[3] = ((R)1 - alpha) * a0.w + alpha * a1.w;
+            }
         }
     } else {
 #pragma unroll
@@ -244,8 +253,9 @@ __global__ __launch_bounds__(kWG) void k_fill_c2f(LevelDev Lf, const uint32_t* _
     R f[Q], nm[24];
 #pragma unroll
     for (int i = 0; i < Q; ++i) f[i] = __shfl(pv[i >> 3], i & 7, 8);
+    // nm[4a + c]: moment c of neighbour a, held by lane a
 #pragma unroll
-    for (int j = 0; j < 24; ++j) nm[j] = __shfl(mv[j >> 3], j & 7, 8);
+    for (int j = 0; j < 24; ++j) nm[j] = __shfl(mv[j & 3], j >> 2, 8);
     if (!live) return;
     const R fac = (R)c2f_factor(tau0, Lf.level);
     R mp[4];

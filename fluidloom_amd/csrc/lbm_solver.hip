@@ -763,7 +763,7 @@ uint64_t layout_bytes(const fl_lbm_solver* S) {
               H.force_links.size() + H.abeg.size() + H.lbeg.size()) * 4;
         t += (uint64_t)NI * (sizeof(float) + sizeof(double));
         t += std::max<uint64_t>((uint64_t)H.nactive * 4, (uint64_t)H.force_links.size() * 3) * 8;
-        t += 2ull * H.nb * 64 * Q * sb + 2ull * H.cap_slots.size() * kCapPlanes * cb;
+        t += 2ull * H.nb * 64 * Q * sb + 2ull * cap_elems(H.cap_slots.size()) * cb;
     }
     return t;
 }
@@ -812,7 +812,7 @@ fl_status upload_and_init(fl_lbm_solver* S) {
         const size_t slots = (size_t)H.nb * 64;
         for (int k = 0; k < 2; ++k) {
             if ((r = dev_alloc(S, &D.buf[k], slots * Q * sb))) return r;
-            if ((r = dev_alloc(S, &D.cap[k], H.cap_slots.size() * kCapPlanes * cb))) return r;
+            if ((r = dev_alloc(S, &D.cap[k], cap_elems(H.cap_slots.size()) * cb))) return r;
         }
         D.cur = 0;
         D.ld.nbr = D.nbr;
