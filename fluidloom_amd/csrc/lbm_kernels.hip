@@ -228,10 +228,16 @@ __global__ __launch_bounds__(kWG) void k_fill_c2f(LevelDev Lf, const uint32_t* _
             live = I.live != 0;
             tau0 = I.tau0;
         }
+        // alpha == 0 (the first fine sub-step of a coarse step) needs only the
+        // earlier capture: the same values, half the loads
+        const bool two = alpha != (R)0;
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             const int i = q + 8 * r;  // population i of P
-            if (i < Q) pv[r] = ((R)1 - alpha) * cap0[(size_t)i * ncap + ci[0]] + alpha * cap1[(size_t)i * ncap + ci[0]];
+            if (i < Q) {
+                const R a0 = cap0[(size_t)i * ncap + ci[0]];
+                pv[r] = two ? ((R)1 - alpha) * a0 + alpha * cap1[(size_t)i * ncap + ci[0]] : a0;
+            }
         }
         // lane a < 6: the moment record of neighbour a, one access per capture
         if (q < 6) {
@@ -239,11 +245,18 @@ __global__ __launch_bounds__(kWG) void k_fill_c2f(LevelDev Lf, const uint32_t* _
             if (c != kNoCap) {
                 typedef R R4 __attribute__((ext_vector_type(4)));
                 const R4 a0 = *reinterpret_cast<const R4*>(cap0 + cap_moments(ncap) + 4 * (size_t)c);
-                const R4 a1 = *reinterpret_cast<const R4*>(cap1 + cap_moments(ncap) + 4 * (size_t)c);
-                mv[0] = ((R)1 - alpha) * a0.x + alpha * a1.x;
-                mv[1] = ((R)1 - alpha) * a0.y + alpha * a1.y;
-                mv[2] = ((R)1 - alpha) * a0.z + alpha * a1.z;
-                mv[3] = ((R)1 - alpha) * a0.w + alpha * a1.w;
+                if (two) {
+                    const R4 a1 = *reinterpret_cast<const R4*>(cap1 + cap_moments(ncap) + 4 * (size_t)c);
+                    mv[0] = ((R)1 - alpha) * a0.x + alpha * a1.x;
+                    mv[1] = ((R)1 - alpha) * a0.y + alpha * a1.y;
+                    mv[2] = ((R)1 - alpha) * a0.z + alpha * a1.z;
+                    mv[3] = ((R)1 - alpha) * a0.w + alpha * a1.w;
+                } else {
+                    mv[0] = a0.x;
+                    mv[1] = a0.y;
+                    mv[2] = a0.z;
+                    mv[3] = a0.w;
+                }
             }
         }
     } else {
