@@ -175,9 +175,13 @@ __global__ __launch_bounds__(kBlock) void k_face_nbr(CellsC c, size_t n, const R
 }
 
 // ---------------------------------------------------------------- a12
-__global__ __launch_bounds__(kBlock) void k_shadow(CellsC c, size_t n, uint8_t* __restrict__ shadow) {
+// shadow = level + (flag > 0) (update_shadow_levels, BalanceEnforcer.cpp:53-62);
+// with_flags = 0 gives the plain copy of the stored levels that the reference
+// loop starts from (BalanceEnforcer.cpp:165-167).
+__global__ __launch_bounds__(kBlock) void k_shadow(CellsC c, size_t n, int with_flags,
+                                                   uint8_t* __restrict__ shadow) {
     for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
-        shadow[i] = (uint8_t)(c.level[i] + (c.flag[i] > 0 ? 1 : 0));
+        shadow[i] = (uint8_t)(c.level[i] + (with_flags && c.flag[i] > 0 ? 1 : 0));
 }
 
 // mode-0 key: anchor (finest units of level M <= 19) with the holder's shadow level
@@ -718,7 +722,10 @@ fl_status run_balance(const fl_regrid_config* cfg, fl_cells* cells, size_t n, co
         fl_status st = build_cell_hash(c, n, keys, tk, tv, x.L.cap, x.s);
         if (st) return st;
     }
-    hipLaunchKernelGGL(k_shadow, dim3(sgrid(n)), dim3(kBlock), 0, x.s, c, n, shadow);
+    // mode 0 starts from the stored levels; flags set before the call enter
+    // the shadow only after the first marking pass, as in the reference
+    hipLaunchKernelGGL(k_shadow, dim3(sgrid(n)), dim3(kBlock), 0, x.s, c, n, cfg->balance_mode != 0 ? 1 : 0,
+                       shadow);
     FL_CHECK_LAUNCH();
     for (int it = 0; it < cfg->max_balance_iters; ++it) {
         FL_HIP(hipMemsetAsync(cnt, 0, 8, x.s));
@@ -740,7 +747,7 @@ fl_status run_balance(const fl_regrid_config* cfg, fl_cells* cells, size_t n, co
         hipLaunchKernelGGL(k_mark, dim3(sgrid(n)), dim3(kBlock), 0, x.s, cw, n, cfg->max_split_level, viol,
                            shadow, cnt + 1);
         FL_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_shadow, dim3(sgrid(n)), dim3(kBlock), 0, x.s, c, n, shadow);
+        hipLaunchKernelGGL(k_shadow, dim3(sgrid(n)), dim3(kBlock), 0, x.s, c, n, 1, shadow);
         FL_CHECK_LAUNCH();
         uint32_t hc[2];
         FL_HIP(hipMemcpyAsync(hc, cnt, 8, hipMemcpyDeviceToHost, x.s));
@@ -769,6 +776,7 @@ fl_status fl_face_neighbors(const fl_cells* c, size_t n, const int32_t* root, ui
     if (n == 0) return FL_OK;
     FL_REQUIRE(c && c->x && c->y && c->z && c->level && d_keys_scratch && d_tkeys && d_tvals && d_nbr && root,
                "null pointer");
+    FL_REQUIRE(n <= kMaxMapCells, "too many cells for the cell map (at most 644,245,094)");
     FL_REQUIRE(capacity >= fl_hash_capacity(n) && (capacity & (capacity - 1)) == 0,
                "capacity must be a power of two >= fl_hash_capacity(n)");
     hipStream_t st = FL_STREAM(s);
@@ -849,6 +857,7 @@ size_t fl_regrid_temp_bytes(size_t n) { return regrid_layout(n > 0 ? n : 1).tota
 fl_status fl_balance(const fl_regrid_config* cfg, fl_cells* c, size_t n, void* d_temp, size_t temp_bytes,
                      fl_regrid_result* res, fl_stream s) {
     FL_REQUIRE(cfg && c && res, "null pointer");
+    FL_REQUIRE(n <= kMaxMapCells, "too many cells for the cell map (at most 644,245,094)");
     RegridLayout L = regrid_layout(n > 0 ? n : 1);
     FL_REQUIRE(n == 0 || temp_bytes >= L.total, "temp buffer too small (see fl_regrid_temp_bytes)");
     Ctx x{static_cast<char*>(d_temp), L, FL_STREAM(s)};
@@ -883,6 +892,7 @@ fl_status fl_split(const fl_cells* c, size_t n, int max_split_level, uint32_t* d
 fl_status fl_merge(const fl_cells* c, size_t n, uint32_t* d_group_id, uint32_t* d_group_leader, fl_cells* parents,
                    size_t parent_capacity, void* d_temp, size_t temp_bytes, uint64_t* h_num_groups, fl_stream s) {
     FL_REQUIRE(c && parents && h_num_groups && d_group_id && d_group_leader, "null pointer");
+    FL_REQUIRE(n <= kMaxMapCells, "too many cells for the cell map (at most 644,245,094)");
     RegridLayout L = regrid_layout(n > 0 ? n : 1);
     FL_REQUIRE(temp_bytes >= L.total, "temp buffer too small (see fl_regrid_temp_bytes)");
     Ctx x{static_cast<char*>(d_temp), L, FL_STREAM(s)};
@@ -922,6 +932,7 @@ fl_status fl_regrid(const fl_regrid_config* cfg, fl_cells* in, size_t n, fl_cell
                     fl_regrid_result* res, fl_stream s) {
     FL_REQUIRE(cfg && in && out && res, "null pointer");
     FL_REQUIRE(cfg->ncomp == 0 || (d_fields_in && d_fields_out), "fields required when ncomp > 0");
+    FL_REQUIRE(n <= kMaxMapCells, "too many cells for the cell map (at most 644,245,094)");
     RegridLayout L = regrid_layout(n > 0 ? n : 1);
     FL_REQUIRE(temp_bytes >= L.total, "temp buffer too small (see fl_regrid_temp_bytes)");
     std::memset(res, 0, sizeof(*res));
@@ -943,7 +954,7 @@ fl_status fl_regrid(const fl_regrid_config* cfg, fl_cells* in, size_t n, fl_cell
         uint32_t* gid = x.at<uint32_t>(L.gid);
         uint32_t* leaders = x.at<uint32_t>(L.leaders);
         uint8_t* shadow = x.at<uint8_t>(L.shadow);
-        hipLaunchKernelGGL(k_shadow, dim3(sgrid(n)), dim3(kBlock), 0, st, c, n, shadow);
+        hipLaunchKernelGGL(k_shadow, dim3(sgrid(n)), dim3(kBlock), 0, st, c, n, 1, shadow);
         hipLaunchKernelGGL(k_fill_u32, dim3(sgrid(n)), dim3(kBlock), 0, st, gid, n, kInvalid);
         hipLaunchKernelGGL(k_merge, dim3((unsigned)scan_tiles(n)), dim3(kScanThreads), 0, st, c, n, h, shadow,
                            cfg->merge_balance_check, Root{{cfg->root[0], cfg->root[1], cfg->root[2]}}, gid, leaders, wview(out), (size_t)0,

@@ -18,6 +18,9 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
 constexpr uint64_t kEmptyKey = 0xFFFFFFFFFFFFFFFFull;
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 constexpr uint32_t kFiner = 0xFFFFFFFEu;
+// Largest cell count an exact cell map holds at load <= 0.6 within the
+// capacity rule's 2^30-slot ceiling: floor(0.6 * 2^30).
+constexpr size_t kMaxMapCells = 644245094;
 
 inline unsigned grid_for(size_t n, unsigned block) {
     size_t g = (n + block - 1) / block;

@@ -71,10 +71,15 @@ fl_status hash_build_impl(uint64_t* tk, uint32_t* tv, uint64_t cap, const uint64
     return FL_OK;
 }
 
-// exact cell map (value = smallest index holding the key) for the AMR kernels
+// exact cell map (value = smallest index holding the key) for the AMR kernels.
+// Its lookups stop at the first empty slot, so the load must stay <= 0.6:
+// with the capacity rule's 2^30 ceiling that bounds n by kMaxMapCells (which
+// also keeps cell indices inside u32 below the kFiner / kInvalid markers).
 fl_status cellmap_build_impl(uint64_t* tk, uint32_t* tv, uint64_t cap, const uint64_t* keys, size_t n,
                              hipStream_t s) {
     if (n == 0) return FL_OK;
+    FL_REQUIRE(n <= kMaxMapCells, "too many cells for the cell map (at most 644,245,094)");
+    FL_REQUIRE((double)n <= 0.6 * (double)cap, "cell map capacity below n / 0.6");
     hipLaunchKernelGGL(k_cell_build, dim3(cap_grid(n)), dim3(kBlock), 0, s, tk, tv, cap, keys, n);
     FL_CHECK_LAUNCH();
     return FL_OK;

@@ -295,12 +295,21 @@ __device__ inline R special_link(const LevelDev& L, const T* __restrict__ A, int
         }
         return Store<T>::load(A, pidx(opp(i), L.S, slot), opp(i)) + (R)6 * (R)wq(i) * (R)I.rho0 * uw;
     }
-    const int ex = I.ext0[0] << L.level, ey = I.ext0[1] << L.level, ez = I.ext0[2] << L.level;
-    const int gx = c3[0] * 4 + px, gy = c3[1] * 4 + py, gz = c3[2] * 4 + pz;
-    const int qx = gx < 0 ? 0 : (gx >= ex ? ex - 1 : gx);
-    const int qy = gy < 0 ? 0 : (gy >= ey ? ey - 1 : gy);
-    const int qz = gz < 0 ? 0 : (gz >= ez ? ez - 1 : gz);
-    const int64_t s2 = slot_of(L, b, qx - c3[0] * 4, qy - c3[1] * 4, qz - c3[2] * 4);
+    // outflow: the source clamped into the domain along the axes it leaves
+    // through a non-periodic face; along a periodic axis the offset stays
+    // as it is and the brick neighbour table (which wraps periodic faces)
+    // resolves the image, as the layout's source_of does
+    const int p[3] = {px, py, pz};
+    int r[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int e = I.ext0[a] << L.level;
+        const int g = c3[a] * 4 + p[a];
+        r[a] = p[a];
+        if (g < 0 && I.bc_type[2 * a] != FL_BC_PERIODIC) r[a] = -c3[a] * 4;
+        else if (g >= e && I.bc_type[2 * a + 1] != FL_BC_PERIODIC) r[a] = e - 1 - c3[a] * 4;
+    }
+    const int64_t s2 = slot_of(L, b, r[0], r[1], r[2]);
     return Store<T>::load(A, pidx(i, L.S, (size_t)s2), i);
 }
 

@@ -549,7 +549,12 @@ int or_balance(const or_cells* c, int mode, int anchor_level, int max_split_leve
         cell_keys(c, keys);
         cm_build(&m, c, keys);
     }
-    for (size_t i = 0; i < n; ++i) shadow[i] = (uint8_t)(c->level[i] + (c->flag[i] > 0 ? 1 : 0));
+    /* Mode 0 starts from a copy of the stored levels (BalanceEnforcer.cpp:
+     * 165-167): split flags set before the call count only after the first
+     * marking pass re-derives the shadow (:248-254). Mode 1, the lattice rule,
+     * counts the requested splits from the start. */
+    for (size_t i = 0; i < n; ++i)
+        shadow[i] = (uint8_t)(c->level[i] + (mode != 0 && c->flag[i] > 0 ? 1 : 0));
     int it = 0;
     *converged = 0;
     *total_violations = 0;

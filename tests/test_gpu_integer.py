@@ -187,6 +187,42 @@ def test_regrid_reference_mode_matches_oracle(fl, oracle):
     assert st.balance_converged == bool(res[1]) and st.balance_violations == res[2] and st.balance_marked == res[3]
 
 
+def test_balance_reference_mode_preflagged_on_gpu(fl, oracle):
+    """The reference's balance loop starts from the stored levels and counts
+    split flags set before the call only after its first marking pass
+    (BalanceEnforcer.cpp:165-167, :248-254).  The two hand-derived cases of
+    the oracle tests, then random complete octrees with 15 % pre-set flags,
+    whose balance cascades over several passes."""
+    from fluidloom_amd import amr
+    from tests.amr_util import random_octree
+    cases = [(dict(x=[0, 2], y=[0, 0], level=[1, 2], flag=[0, 1]), (1, 1, 0, 0), [0, 1]),
+             (dict(x=[0, 2, 0, 4], y=[0, 0, 1, 4], level=[1, 2, 1, 3], flag=[0, 1, 0, 0]), (3, 1, 2, 2),
+              [1, 1, 1, 0])]
+    for geo, exp, flags in cases:
+        n = len(geo["x"])
+        c = oracle.CellSet(n, **geo)
+        g = _gpu_cells(c)
+        out, st = amr.regrid(g, balance_mode=0, anchor_level=8, max_split_level=8, max_iters=10)
+        assert (st.balance_iterations, int(st.balance_converged), st.balance_violations, st.balance_marked) == exp
+        assert st.num_split == sum(flags)
+        ref, res, _, _ = oracle.regrid(c, 0, 8, 8, 10)
+        assert res == exp
+        _same_cells(out.to_host(), ref)
+    multi = 0
+    for seed in (5, 6, 7, 21, 22):
+        c = random_octree(seed=seed, root=(2, 2, 2), depth=3, p_split=0.5)
+        c.level[:] += 1
+        rng = np.random.default_rng(seed)
+        c.flag[:] = np.where(rng.random(c.n) < 0.15, 1, 0)
+        g = _gpu_cells(c)
+        out, st = amr.regrid(g, balance_mode=0, anchor_level=6, max_split_level=6, max_iters=64)
+        ref, res, _, _ = oracle.regrid(c, 0, 6, 6, 64)
+        _same_cells(out.to_host(), ref)
+        assert (st.balance_iterations, int(st.balance_converged), st.balance_violations, st.balance_marked) == res
+        multi += res[0] > 1
+    assert multi >= 2
+
+
 def test_regrid_lbm_mode_matches_oracle(fl, oracle):
     from fluidloom_amd import amr
     from tests.amr_util import random_octree

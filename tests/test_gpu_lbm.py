@@ -8,10 +8,10 @@ relative L2."""
 import numpy as np
 import pytest
 
-from tests.lbm_cases import case, gpu_solver, oracle_solver, perturb
+from tests.lbm_cases import case, couette_wall, gpu_solver, oracle_solver, perturb
 
 pytestmark = pytest.mark.gpu
-CASES = ["periodic", "cavity", "cavity2", "patch2", "sphere3"]
+CASES = ["periodic", "cavity", "cavity2", "patch2", "sphere3", "outper2"]
 
 
 def rel_l2(a, b):
@@ -182,4 +182,47 @@ def test_cavity_full_reference_case_matches_oracle(fl, oracle):
     fo = o.s.get_f()
     fg = g.solver.get_f(o.s.fx, o.s.fy, o.s.fz, o.s.flv)
     assert np.abs(fg - fo).max() / np.abs(fo).max() < 1e-10
+    assert (g.solver.level == 1).any() and g.regrids == 9
+
+
+@pytest.mark.parametrize("refined", [False, True])
+def test_momentum_exchange_force_couette_on_gpu(fl, oracle, refined):
+    """Steady plane Couette flow over a stationary wall of solid cells: the
+    device force equals the closed form F = (rho nu U / H A, 0, -rho cs^2 A)
+    (exact for half-way bounce-back; the oracle holds it to round-off).
+    Tolerances: 1e-6 relative in FP64 storage, 1e-4 in FP32."""
+    cs, exact = couette_wall(oracle, 8, 1.0, refined)
+    for storage, tol in (("f64", 1e-6), ("f32", 1e-4)):
+        g = gpu_solver(cs, storage)
+        g.step(3000 if refined else 2000)
+        F = np.asarray(g.force())
+        g.close()
+        assert abs(F[0] / exact[0] - 1) < tol, (storage, F, exact)
+        assert abs(F[1]) < tol * abs(exact[0]) and abs(F[2] / exact[2] - 1) < tol, (storage, F, exact)
+
+
+def test_cavity_full_reference_case_f32_fields(fl, oracle):
+    """The same reference case in FP32 storage (the configuration's type):
+    32^3, 500 steps, 9 regrids.  Every regrid gives the oracle's mesh, and
+    density and velocity end within 1e-4 relative L2 of the FP64 oracle."""
+    from fluidloom_amd.cavity import Cavity
+    from tests.cavity_util import OracleCavity
+    g = Cavity(n0=32, max_level=1, storage="f32")
+    o = OracleCavity(oracle, n0=32, max_level=1)
+    for chunk in range(10):
+        g.run(50, 0)
+        o.run(50, 0)
+        if chunk < 9:
+            g.regrid()
+            o.regrid()
+            gs, os_ = g.solver, o.s
+            ka = np.lexsort((gs.x, gs.y, gs.z, gs.level))
+            kb = np.lexsort((os_.x, os_.y, os_.z, os_.level))
+            for a, b in [(gs.x, os_.x), (gs.y, os_.y), (gs.z, os_.z), (gs.level, os_.level)]:
+                assert np.array_equal(a[ka], b[kb]), chunk
+    ro, uo = o.s.moments()
+    rg, ug = g.solver.moments(o.s.fx, o.s.fy, o.s.fz, o.s.flv)
+    e_rho, e_u = rel_l2(rg, ro), rel_l2(ug, uo)
+    print(f"C1 FP32 vs FP64 oracle after 500 steps: rho {e_rho:.2e} u {e_u:.2e}", flush=True)
+    assert e_rho < 1e-4 and e_u < 1e-4
     assert (g.solver.level == 1).any() and g.regrids == 9

@@ -37,6 +37,31 @@ def test_error_convention_without_device():
     assert L.fl_hash_capacity(700000) == 1 << 21
 
 
+def test_cell_map_size_bound_without_device():
+    """The exact cell map needs load <= 0.6 inside its 2^30-slot ceiling:
+    more than floor(0.6 * 2^30) cells are rejected before any device work
+    (the pointers below are never dereferenced)."""
+    import ctypes as C
+
+    from fluidloom_amd import _lib
+    L = _lib.lib()
+    big = 644245095
+    dummy = C.c_void_p(16)
+    cells = _lib.Cells(*([dummy.value] * 7))
+    cfg = _lib.RegridConfig(balance_mode=1, anchor_level=10, max_split_level=10, max_balance_iters=8,
+                            root=(_lib.i32 * 3)(1, 1, 1))
+    res = _lib.RegridResult()
+    tb = L.fl_regrid_temp_bytes(big)
+    st = L.fl_regrid(C.byref(cfg), C.byref(cells), big, C.byref(cells), big, None, None, dummy, tb, C.byref(res),
+                     None)
+    assert st == _lib.FL_E_INVALID and "cell map" in L.fl_last_error_string().decode()
+    st = L.fl_balance(C.byref(cfg), C.byref(cells), big, dummy, tb, C.byref(res), None)
+    assert st == _lib.FL_E_INVALID and "cell map" in L.fl_last_error_string().decode()
+    root = (_lib.i32 * 3)(1, 1, 1)
+    st = L.fl_face_neighbors(C.byref(cells), big, root, dummy, dummy, dummy, 1 << 30, dummy, None)
+    assert st == _lib.FL_E_INVALID and "cell map" in L.fl_last_error_string().decode()
+
+
 def test_plan_bytes_host_only(oracle):
     """fl_lbm_plan_bytes runs the layout on the host (no GPU needed) and
     reports the footprint a batch would allocate; mesh errors surface here."""

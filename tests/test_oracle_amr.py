@@ -40,6 +40,43 @@ def test_balance_fixture(oracle):
     assert c.flag.tolist() == [1, 0]
 
 
+def test_balance_preflagged_hand_derived(oracle):
+    """Expected values stepped through BalanceEnforcer.cpp:145-263 and
+    balance_enforce.cl:57-172 by hand, anchor level 8 (MAX_REFINEMENT_LEVEL).
+
+    A = (x 0, level 1), B = (x 2, level 2, flag 1).  Iteration 1 runs on a
+    copy of the stored levels: A's +x test point 128 is first held at probe
+    level 2 (B), and 2 > 1 + 1 is false; B's -x point 127 is held by A at
+    level 1.  No violation: 1 iteration, converged, nothing marked."""
+    c = oracle.CellSet(2, x=[0, 2], level=[1, 2], flag=[0, 1])
+    assert oracle.balance(c, 0, 8, 8, 10) == (1, 1, 0, 0)
+    assert c.flag.tolist() == [0, 1]
+
+
+def test_balance_preflag_enters_after_first_marking(oracle):
+    """Hand-derived from the same reference loop.  A = (0,0,0 L1),
+    B = (2,0,0 L2, flag 1), C = (0,1,0 L1), E = (4,4,0 L3); anchors in 2^8
+    units: A 0, B (128,0,0), C (0,128,0), E (128,128,0).
+
+    it 1 (shadow = stored levels 1,2,1,3): C's +x point (128,128,0) is first
+    held at probe level 3, by E, and 3 > 1 + 1: C violates and is marked.
+    A's +x point (128,0,0) is held at probe level 2 by B: no violation.
+    Shadow becomes 1,3,2,3 (B's own flag now counts).
+    it 2: A's +x point is now held at probe level 3 by B: A violates and is
+    marked.  Shadow 2,3,2,3.
+    it 3: no test point finds a holder more than one level finer: converged.
+    Result (3 iterations, converged, 2 violations, 2 marked), flags 1,1,1,0.
+    Seeding the shadow with the flags from the start (the rule of the lattice
+    mode) would mark A and C together in iteration 1 and stop after 2."""
+    geo = dict(x=[0, 2, 0, 4], y=[0, 0, 1, 4], level=[1, 2, 1, 3], flag=[0, 1, 0, 0])
+    c = oracle.CellSet(4, **geo)
+    assert oracle.balance(c, 0, 8, 8, 10) == (3, 1, 2, 2)
+    assert c.flag.tolist() == [1, 1, 1, 0]
+    c = oracle.CellSet(4, **geo)
+    assert oracle.balance(c, 0, 8, 8, 1) == (1, 0, 1, 1)
+    assert c.flag.tolist() == [0, 1, 1, 0]
+
+
 def test_face_neighbors_vs_brute_force(oracle):
     c = random_octree(seed=1, root=(2, 2, 2), depth=3)
     got = oracle.face_neighbors(c, (2, 2, 2)).reshape(-1, 6)
@@ -78,8 +115,11 @@ def _brute_balance(c, mode, M, max_split, max_iters=64, root=(1, 1, 1)):
     anchors = [(int(X[j]) << (M - int(L[j])), int(Y[j]) << (M - int(L[j])), int(Z[j]) << (M - int(L[j])))
                if L[j] <= M else None for j in range(n)]
     it = conv = tv = tm = 0
+    # the reference starts from a copy of the stored levels
+    # (BalanceEnforcer.cpp:165-167) and adds the split flags only after a
+    # marking pass (:248-254); the lattice rule counts them from the start
+    sh = L.copy() if mode == 0 else L + (flag > 0)
     while it < max_iters:
-        sh = L + (flag > 0)
         viol = np.zeros(n, bool)
         for i in range(n):
             if mode == 0:
@@ -127,6 +167,7 @@ def _brute_balance(c, mode, M, max_split, max_iters=64, root=(1, 1, 1)):
         tm += int(mk.sum())
         if mk.sum() == 0:
             break
+        sh = L + (flag > 0)
     return it, conv, tv, tm, flag
 
 
