@@ -105,6 +105,8 @@ _SIGS = {
     "fl_iota_u32": (C.c_int, [vp, sz, vp]),
     "fl_scan_temp_bytes": (sz, [sz]),
     "fl_scan_exclusive_u32": (C.c_int, [vp, vp, vp, sz, vp, sz, vp]),
+    "fl_scan_noalloc_max_elements": (sz, []),
+    "fl_scan_exclusive_u32_noalloc": (C.c_int, [vp, vp, vp, sz, vp]),
     "fl_hash_capacity": (u64, [sz]),
     "fl_hash_clear": (C.c_int, [vp, vp, u64, vp]),
     "fl_hash_build": (C.c_int, [vp, vp, u64, vp, vp, sz, vp, vp]),

@@ -9,6 +9,9 @@ namespace fl {
 namespace {
 thread_local std::string g_last_error;
 }
+// scan.hip: looks up this device's address of the allocation-free scan's
+// scratch array
+fl_status noalloc_scratch_setup(int device, uint64_t** out);
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 
@@ -52,7 +55,7 @@ fl_status fl_init(int device) {
     FL_REQUIRE(device >= 0 && device < n, "device id out of range");
     FL_HIP(hipSetDevice(device));
     FL_HIP(hipFree(nullptr));
-    return FL_OK;
+    return fl::noalloc_scratch_setup(device, nullptr);
 }
 
 fl_status fl_device_count(int* count) {

@@ -115,6 +115,20 @@ size_t fl_scan_temp_bytes(size_t n);
 fl_status fl_scan_exclusive_u32(const uint32_t* d_in, uint32_t* d_out, uint64_t* d_total,
                                 size_t n, void* d_temp, size_t temp_bytes, fl_stream s);
 
+/* Allocation-free form of fl_scan_exclusive_u32: no counterpart in the
+ * reference.  For callers that cannot allocate device memory or pass a
+ * workspace (code under HIP graph capture, tight serving loops).  Same
+ * kernels and results; the look-back workspace is the library's own device
+ * array (1 MiB per device, held until the process ends), so n is limited to
+ * fl_scan_noalloc_max_elements() = 2^31 (larger n: FL_E_INVALID).  Runs on
+ * the current device; call fl_init(device) first so that the array's
+ * address is looked up outside any capture.  Calls on one device share the
+ * array: calls that overlap on one device (other streams or threads) race.
+ * Issues kernels only (no allocation, no memset). */
+size_t fl_scan_noalloc_max_elements(void);
+fl_status fl_scan_exclusive_u32_noalloc(const uint32_t* d_in, uint32_t* d_out, uint64_t* d_total,
+                                        size_t n, fl_stream s);
+
 /* ---- a7: GPU hash table ---------------------------------------------------
  * Replaces hash_clear/hash_build/hash_query (kernels/hashmap/hash_build.cl:19-82,
  * hash_query.cl:22-98) behind HashTableManager (HashTableManager.cpp:47-159).

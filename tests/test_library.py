@@ -35,6 +35,10 @@ def test_error_convention_without_device():
     assert st == _lib.FL_E_INVALID
     assert "null" in L.fl_last_error_string().decode()
     assert L.fl_hash_capacity(700000) == 1 << 21
+    # the allocation-free scan rejects sizes beyond its scratch array first
+    assert L.fl_scan_noalloc_max_elements() == 1 << 31
+    st = L.fl_scan_exclusive_u32_noalloc(None, None, None, (1 << 31) + 1, None)
+    assert st == _lib.FL_E_INVALID and "2^31" in L.fl_last_error_string().decode()
 
 
 def test_cell_map_size_bound_without_device():
