@@ -156,6 +156,7 @@ _SIGS = {
     "fl_lbm_totals_batch": (C.c_int, [vp, vp, vp]),
     "fl_lbm_force_batch": (C.c_int, [vp, vp]),
     "fl_lbm_velocity_change": (C.c_int, [vp, vp]),
+    "fl_lbm_velocity_sample": (C.c_int, [vp]),
     "fl_lbm_set_live": (C.c_int, [vp, C.c_int, C.c_int]),
     "fl_lbm_set_graph": (C.c_int, [vp, C.c_int]),
     "fl_lbm_num_variants": (C.c_int, []),

@@ -88,12 +88,23 @@ template <typename T>
 struct Store;
 // store_nt: streaming store of the collide output (not re-read by the same
 // launch), kept out of the caches that serve neighbour re-reads.
+//
+// Besides the population itself (load / store), every codec reads and writes
+// its deviation h = f - w_q (dev / store_dev*), which the stepping kernels
+// compute with: density and momentum are then sums of small numbers
+// (rho = 1 + sum h), so neither the rounding of the weights to the storage
+// type nor the fixed low bits that 16-bit storage leaves in w + h bias them.
 template <>
 struct Store<float> {
     using R = float;
     __device__ static inline float load(const float* p, size_t i, int) { return p[i]; }
     __device__ static inline void store(float* p, size_t i, int, float v) { p[i] = v; }
     __device__ static inline void store_nt(float* p, size_t i, int, float v) { __builtin_nontemporal_store(v, p + i); }
+    __device__ static inline float dev(const float* p, size_t i, int q) { return p[i] - (float)wq(q); }
+    __device__ static inline void store_dev(float* p, size_t i, int q, float h) { p[i] = (float)wq(q) + h; }
+    __device__ static inline void store_dev_nt(float* p, size_t i, int q, float h) {
+        __builtin_nontemporal_store((float)wq(q) + h, p + i);
+    }
 };
 template <>
 struct Store<double> {
@@ -102,6 +113,11 @@ struct Store<double> {
     __device__ static inline void store(double* p, size_t i, int, double v) { p[i] = v; }
     __device__ static inline void store_nt(double* p, size_t i, int, double v) {
         __builtin_nontemporal_store(v, p + i);
+    }
+    __device__ static inline double dev(const double* p, size_t i, int q) { return p[i] - wq(q); }
+    __device__ static inline void store_dev(double* p, size_t i, int q, double h) { p[i] = wq(q) + h; }
+    __device__ static inline void store_dev_nt(double* p, size_t i, int q, double h) {
+        __builtin_nontemporal_store(wq(q) + h, p + i);
     }
 };
 template <>
@@ -124,10 +140,14 @@ struct Store<_Float16> {
     // below-subnormal inputs).  Keeps slow updates (below half an FP16 ulp per
     // step) from being lost, which round-to-nearest does every step.
     __device__ static inline void store_sr(_Float16* p, size_t i, int q, float v, uint32_t r) {
+        store_dev_sr(p, i, q, v - (float)wq(q), r);
+    }
+    __device__ static inline float dev(const _Float16* p, size_t i, int) { return (float)p[i]; }
+    __device__ static inline void store_dev(_Float16* p, size_t i, int, float h) { p[i] = (_Float16)h; }
+    __device__ static inline void store_dev_sr(_Float16* p, size_t i, int, float h, uint32_t r) {
         typedef _Float16 h2 __attribute__((ext_vector_type(2)));
         const h2 keep = {(_Float16)0, (_Float16)0};
-        const _Float16 h = __builtin_amdgcn_cvt_sr_f16_f32(keep, v - (float)wq(q), r, false)[0];
-        __builtin_nontemporal_store(h, p + i);
+        __builtin_nontemporal_store(__builtin_amdgcn_cvt_sr_f16_f32(keep, h, r, false)[0], p + i);
     }
 };
 
@@ -148,7 +168,17 @@ struct Store<fp16c> {
     }
     // (f - w) (2^-113 / w): for |x| >= 2^-13 the same rounding as (f - w)(1 / w)
     __device__ static inline void store_sr(fp16c* p, size_t i, int q, float v, uint32_t r) {
-        const float xs = (v - (float)wq(q)) * ((float)(1.0 / wq(q)) * kFp16cDown);
+        store_dev_sr(p, i, q, v - (float)wq(q), r);
+    }
+    // h = w x: the code float times 2^113 w
+    __device__ static inline float dev(const fp16c* p, size_t i, int q) {
+        return fp16c_code_float(p[i].bits) * ((float)wq(q) * kFp16cUp);
+    }
+    __device__ static inline void store_dev(fp16c* p, size_t i, int q, float h) {
+        p[i].bits = fp16c_encode_rne(h * (float)(1.0 / wq(q)));
+    }
+    __device__ static inline void store_dev_sr(fp16c* p, size_t i, int q, float h, uint32_t r) {
+        const float xs = h * ((float)(1.0 / wq(q)) * kFp16cDown);
         __builtin_nontemporal_store(fp16c_encode_sr_scaled(xs, r), &p[i].bits);
     }
 };
@@ -204,6 +234,43 @@ __device__ inline void bgk(R* f, R inv_tau) {
     const R usq = ux * ux + uy * uy + uz * uz;
 #pragma unroll
     for (int i = 0; i < Q; ++i) f[i] = f[i] - (f[i] - feq(i, rho, ux, uy, uz, usq)) * inv_tau;
+}
+
+// ---- deviation form: h_i = f_i - w_i --------------------------------------
+// Equilibrium deviation feq_i - w_i = w_i (d + rho (3 cu + 4.5 cu^2 - 1.5 u^2))
+// with d = rho - 1 (from sum_i w_i = 1 and sum_i w_i c_i = 0).
+template <typename R>
+__device__ inline R feq_dev(int i, R d, R rho, R ux, R uy, R uz, R usq) {
+    const R cu = (R)cx(i) * ux + (R)cy(i) * uy + (R)cz(i) * uz;
+    return (R)wq(i) * (d + rho * ((R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * usq));
+}
+
+// d = sum h = rho - 1, j = sum c h (= sum c f)
+template <typename R>
+__device__ inline void dev_density_momentum(const R* h, R& d, R& jx, R& jy, R& jz) {
+    density_momentum(h, d, jx, jy, jz);
+}
+
+template <typename R>
+__device__ inline void dev_moments(const R* h, R& d, R& rho, R& ux, R& uy, R& uz) {
+    R jx, jy, jz;
+    density_momentum(h, d, jx, jy, jz);
+    rho = (R)1 + d;
+    const R ir = (R)1 / rho;
+    ux = jx * ir;
+    uy = jy * ir;
+    uz = jz * ir;
+}
+
+// BGK on deviations: h* = h - (h - (feq - w)) / tau (in place); the same
+// update as bgk() on f = w + h.
+template <typename R>
+__device__ inline void bgk_dev(R* h, R inv_tau) {
+    R d, rho, ux, uy, uz;
+    dev_moments(h, d, rho, ux, uy, uz);
+    const R usq = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) h[i] = h[i] - (h[i] - feq_dev(i, d, rho, ux, uy, uz, usq)) * inv_tau;
 }
 
 // Per-cell link codes (2 bits per direction in a u64; bit 63 = active fluid
@@ -313,6 +380,40 @@ __device__ inline R special_link(const LevelDev& L, const T* __restrict__ A, int
     return Store<T>::load(A, pidx(i, L.S, (size_t)s2), i);
 }
 
+// special_link on deviations: w_opp(i) = w_i, so bounce-back and the wall
+// term carry over unchanged.
+template <typename T, typename R>
+__device__ inline R special_link_dev(const LevelDev& L, const T* __restrict__ A, int b, size_t slot, int i,
+                                     unsigned code, int px, int py, int pz) {
+    if (code == kLinkSolid) return Store<T>::dev(A, pidx(opp(i), L.S, slot), opp(i));
+    const InstDev& I = L.inst[L.binst[b]];
+    const int32_t* c3 = L.bcoord + 3 * b;
+    if (code == kLinkWall) {
+        const int face = wall_face(I, L.level, c3[0] * 4 + px, c3[1] * 4 + py, c3[2] * 4 + pz);
+        R uw = 0;
+        if (face >= 0) {
+            if (sizeof(R) == 8)
+                uw = (R)(cx(i) * I.bc_ud[face][0] + cy(i) * I.bc_ud[face][1] + cz(i) * I.bc_ud[face][2]);
+            else
+                uw = (R)((float)cx(i) * I.bc_u[face][0] + (float)cy(i) * I.bc_u[face][1] +
+                         (float)cz(i) * I.bc_u[face][2]);
+        }
+        return Store<T>::dev(A, pidx(opp(i), L.S, slot), opp(i)) + (R)6 * (R)wq(i) * (R)I.rho0 * uw;
+    }
+    const int p[3] = {px, py, pz};
+    int r[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int e = I.ext0[a] << L.level;
+        const int g = c3[a] * 4 + p[a];
+        r[a] = p[a];
+        if (g < 0 && I.bc_type[2 * a] != FL_BC_PERIODIC) r[a] = -c3[a] * 4;
+        else if (g >= e && I.bc_type[2 * a + 1] != FL_BC_PERIODIC) r[a] = e - 1 - c3[a] * 4;
+    }
+    const int64_t s2 = slot_of(L, b, r[0], r[1], r[2]);
+    return Store<T>::dev(A, pidx(i, L.S, (size_t)s2), i);
+}
+
 // Offset of the source that cell (lx, ly, lz) of a brick pulls in direction
 // i, from plane i's base (A + plane_base(i, S)).  NB: the brick's 27
 // neighbour entries (13 = the brick itself); the source brick is one of at
@@ -364,6 +465,37 @@ __device__ inline void gather_slot(const LevelDev& L, const T* __restrict__ A, i
 template <typename T, typename R>
 __device__ inline void gather_slot(const LevelDev& L, const T* __restrict__ A, int64_t slot, R* f) {
     gather_slot<T, R>(L, A, slot, L.cflags[slot], f);
+}
+
+// gather_slot on deviations h = f - w (the fills' and captures' input).
+template <typename T, typename R>
+__device__ inline void gather_slot_dev(const LevelDev& L, const T* __restrict__ A, int64_t slot, uint64_t w, R* h) {
+    const int b = (int)(slot >> 6);
+    const int l = (int)(slot & 63);
+    const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
+    if ((w & ((1ull << (2 * Q)) - 1)) == 0) {
+        const int* __restrict__ row = L.nbr + (size_t)b * 27;
+        int NB[27];
+#pragma unroll
+        for (int k = 0; k < 27; ++k) NB[k] = row[k];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) h[i] = Store<T>::dev(A + plane_base(i, L.S), pull_offset(NB, lx, ly, lz, i), i);
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const unsigned code = (unsigned)(w >> (2 * i)) & 3u;
+        const int px = lx - cx(i), py = ly - cy(i), pz = lz - cz(i);
+        if (code == kLinkRegular)
+            h[i] = Store<T>::dev(A, pidx(i, L.S, (size_t)slot_of(L, b, px, py, pz)), i);
+        else
+            h[i] = special_link_dev<T, R>(L, A, b, (size_t)slot, i, code, px, py, pz);
+    }
+}
+
+template <typename T, typename R>
+__device__ inline void gather_slot_dev(const LevelDev& L, const T* __restrict__ A, int64_t slot, R* h) {
+    gather_slot_dev<T, R>(L, A, slot, L.cflags[slot], h);
 }
 
 }  // namespace lbm

@@ -52,6 +52,31 @@ __device__ inline unsigned xcd_chunk_remap(unsigned bid, unsigned nblk, unsigned
     return ((k / C) * 8u + x) * C + k % C;
 }
 
+// Rounding bits of slot s (brick b = s >> 6, cell s & 63) of a level: the
+// brick's coordinates inside its instance, the cell, the level and the
+// level's step clock, so a ghost rounds the same way alone and in a batch.
+__device__ inline uint32_t slot_bits(const LevelDev& L, size_t s) {
+    const int32_t* c3 = L.bcoord + 3 * (s >> 6);
+    return mix32(mix32((uint32_t)c3[0] * 0x9E3779B1u ^ mix32((uint32_t)c3[1] * 0x85EBCA77u ^
+                                                              mix32((uint32_t)c3[2] * 0xC2B2AE3Du)) ^
+                       ((uint32_t)L.level << 24) ^ (*L.tick * 0x27D4EB2Fu)) +
+                 (uint32_t)(s & 63));
+}
+
+// Ghost population store of the fills, from the deviation h = f - w:
+// stochastic rounding for the 16-bit storages (a ghost is interpolated afresh
+// every sub-step from slowly varying values, so round-to-nearest would feed
+// the fine level the same error at every step), plain stores otherwise.
+// r: the slot's bits; direction i draws r + i * 0x9E3779B9, as the collide
+// does.
+template <typename T>
+__device__ inline void store_ghost(T* p, size_t idx, int i, typename Store<T>::R h, uint32_t r) {
+    if constexpr (std::is_same<T, _Float16>::value || std::is_same<T, fp16c>::value)
+        Store<T>::store_dev_sr(p, idx, i, h, r + (uint32_t)i * 0x9E3779B9u);
+    else
+        Store<T>::store_dev(p, idx, i, h);
+}
+
 template <typename R>
 __device__ inline R inv_tau_of(const LevelDev& L, int inst) {
     if (sizeof(R) == 8) return (R)L.itau_d[inst];
@@ -85,13 +110,21 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
     const size_t S = L.S;
     // pulled source of direction i: offset from plane i's base (A + plane_base)
     auto source = [&](int i) -> unsigned { return pull_offset(NB, lx, ly, lz, i); };
+    // The cell's populations as deviations h = f - w: density and momentum
+    // then come from sums of small numbers, so neither the weights' rounding
+    // to the storage type nor the low bits that 16-bit storage leaves fixed
+    // in every w + h bias them.  16-bit storage reads h directly; FP32 and
+    // FP64 read f and subtract w after the last load has been issued (a
+    // subtraction next to each load made the compiler wait on the loads one
+    // by one: 2.21 ms against 1.74 ms for the sphere's level-3 launch).
+    constexpr bool kDevLoads = sizeof(T) == 2;
     R f[Q];
     if (sizeof(T) == 2 && simple) {
         // 16-bit storage, every cell fluid, every link regular: 19 independent
         // loads back to back.  FP32 and FP64 keep the interleaved form below,
         // which measured faster for FP32.
 #pragma unroll
-        for (int i = 0; i < Q; ++i) f[i] = Store<T>::load(A + plane_base(i, S), source(i), i);
+        for (int i = 0; i < Q; ++i) f[i] = Store<T>::dev(A + plane_base(i, S), source(i), i);
     } else {
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
@@ -99,14 +132,21 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
             if (active) {
                 const unsigned code = (unsigned)(w >> (2 * i)) & 3u;
                 if (code == kLinkRegular)
-                    f[i] = Store<T>::load(A + plane_base(i, S), source(i), i);
+                    f[i] = kDevLoads ? Store<T>::dev(A + plane_base(i, S), source(i), i)
+                                     : Store<T>::load(A + plane_base(i, S), source(i), i);
                 else
-                    f[i] = special_link<T, R>(L, A, b, slot, i, code, lx - cx(i), ly - cy(i), lz - cz(i));
+                    f[i] = kDevLoads ? special_link_dev<T, R>(L, A, b, slot, i, code, lx - cx(i), ly - cy(i),
+                                                              lz - cz(i))
+                                     : special_link<T, R>(L, A, b, slot, i, code, lx - cx(i), ly - cy(i), lz - cz(i));
             }
         }
     }
     if (!active) return;
-    bgk(f, inv_tau_of<R>(L, inst));
+    if (!kDevLoads) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f[i] -= (R)wq(i);
+    }
+    bgk_dev(f, inv_tau_of<R>(L, inst));
     if constexpr (std::is_same<T, _Float16>::value || std::is_same<T, fp16c>::value) {
         // rounding bits from the cell's own coordinates, level and step, so an
         // instance rounds the same way alone and inside any batch.  The
@@ -125,12 +165,13 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
         const uint32_t h = mix32(hb + (uint32_t)lane);
 #pragma unroll
         for (int i = 0; i < Q; ++i)
-            Store<T>::store_sr(B + plane_base(i, S), (unsigned)b * (unsigned)brick_stride(i) + (unsigned)lane, i, f[i],
-                               h + (uint32_t)i * 0x9E3779B9u);
+            Store<T>::store_dev_sr(B + plane_base(i, S), (unsigned)b * (unsigned)brick_stride(i) + (unsigned)lane, i,
+                                   f[i], h + (uint32_t)i * 0x9E3779B9u);
     } else {
 #pragma unroll
         for (int i = 0; i < Q; ++i)
-            Store<T>::store_nt(B + plane_base(i, S), (unsigned)b * (unsigned)brick_stride(i) + (unsigned)lane, i, f[i]);
+            Store<T>::store_dev_nt(B + plane_base(i, S), (unsigned)b * (unsigned)brick_stride(i) + (unsigned)lane, i,
+                                   f[i]);
     }
 }
 
@@ -169,10 +210,11 @@ __global__ __launch_bounds__(kWG) void k_capture(LevelDev L, const T* __restrict
                                                  const uint32_t* __restrict__ cap_slots, uint32_t ncap,
                                                  typename Store<T>::R* __restrict__ cap) {
     using R = typename Store<T>::R;
+    // deviations h = f - w and the record (rho - 1, momentum)
     for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < ncap; k += gridDim.x * kWG) {
         R f[Q], m[4];
-        gather_slot<T, R>(L, A, cap_slots[k], f);
-        density_momentum(f, m[0], m[1], m[2], m[3]);
+        gather_slot_dev<T, R>(L, A, cap_slots[k], f);
+        dev_density_momentum(f, m[0], m[1], m[2], m[3]);
 #pragma unroll
         for (int i = 0; i < Q; ++i) cap[(size_t)i * ncap + k] = f[i];
         typedef R R4 __attribute__((ext_vector_type(4)));
@@ -271,8 +313,10 @@ __global__ __launch_bounds__(kWG) void k_fill_c2f(LevelDev Lf, const uint32_t* _
     for (int j = 0; j < 24; ++j) nm[j] = __shfl(mv[j & 3], j >> 2, 8);
     if (!live) return;
     const R fac = (R)c2f_factor(tau0, Lf.level);
-    R mp[4];
-    moments(f, mp[0], mp[1], mp[2], mp[3]);
+    // f[] holds P's deviations; moments as (rho - 1, u), the neighbours'
+    // records as (rho - 1, momentum)
+    R mp[4], rp;
+    dev_moments(f, mp[0], rp, mp[1], mp[2], mp[3]);
     R phi[4] = {mp[0], mp[1], mp[2], mp[3]};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -280,17 +324,19 @@ __global__ __launch_bounds__(kWG) void k_fill_c2f(LevelDev Lf, const uint32_t* _
         R mm[4] = {0, 0, 0, 0}, mq[4] = {0, 0, 0, 0};
         if (hm) {
             const R* v = nm + 4 * (2 * a);
+            const R ir = (R)1 / ((R)1 + v[0]);
             mm[0] = v[0];
-            mm[1] = v[1] / v[0];
-            mm[2] = v[2] / v[0];
-            mm[3] = v[3] / v[0];
+            mm[1] = v[1] * ir;
+            mm[2] = v[2] * ir;
+            mm[3] = v[3] * ir;
         }
         if (hq) {
             const R* v = nm + 4 * (2 * a + 1);
+            const R ir = (R)1 / ((R)1 + v[0]);
             mq[0] = v[0];
-            mq[1] = v[1] / v[0];
-            mq[2] = v[2] / v[0];
-            mq[3] = v[3] / v[0];
+            mq[1] = v[1] * ir;
+            mq[2] = v[2] * ir;
+            mq[3] = v[3] * ir;
         }
         const R dl = ((q >> a) & 1) ? (R)0.25 : (R)-0.25;
 #pragma unroll
@@ -304,12 +350,15 @@ __global__ __launch_bounds__(kWG) void k_fill_c2f(LevelDev Lf, const uint32_t* _
     }
     const R usq = phi[1] * phi[1] + phi[2] * phi[2] + phi[3] * phi[3];
     const R usp = mp[1] * mp[1] + mp[2] * mp[2] + mp[3] * mp[3];
+    const R rphi = (R)1 + phi[0];
     // only the populations some fluid leaf pulls from this ghost
+    const uint32_t rb = slot_bits(Lf, g);
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         if (!((want >> i) & 1u)) continue;
-        const R e = feq(i, phi[0], phi[1], phi[2], phi[3], usq);
-        Store<T>::store(A, pidx(i, Lf.S, g), i, e + fac * (f[i] - feq(i, mp[0], mp[1], mp[2], mp[3], usp)));
+        const R e = feq_dev(i, phi[0], rphi, phi[1], phi[2], phi[3], usq);
+        store_ghost<T>(A, pidx(i, Lf.S, g), i,
+                       e + fac * (f[i] - feq_dev(i, mp[0], rp, mp[1], mp[2], mp[3], usp)), rb);
     }
 }
 
@@ -371,29 +420,37 @@ __global__ __launch_bounds__(kWG) void k_fill_f2c(LevelDev Lc, const uint32_t* _
 #pragma unroll
     for (int j = 0; j < 4 + Q; ++j) s[j] = 0;
     if (on) {
+        // the child's deviations h = f - w: s = (rho - 1, momentum, f_neq)
         R f[Q];
-        gather_slot<T, R>(Lf, Af, child, w, f);
-        R rho, vx, vy, vz;
-        moments(f, rho, vx, vy, vz);
+        gather_slot_dev<T, R>(Lf, Af, child, w, f);
+        R d, jx, jy, jz;
+        dev_density_momentum(f, d, jx, jy, jz);
+        const R rho = (R)1 + d;
+        const R ir = (R)1 / rho;
+        const R vx = jx * ir, vy = jy * ir, vz = jz * ir;
         const R usq = vx * vx + vy * vy + vz * vz;
-        s[0] = rho;
-        s[1] = rho * vx;
-        s[2] = rho * vy;
-        s[3] = rho * vz;
+        s[0] = d;
+        s[1] = jx;
+        s[2] = jy;
+        s[3] = jz;
 #pragma unroll
-        for (int i = 0; i < Q; ++i) s[4 + i] = f[i] - feq(i, rho, vx, vy, vz, usq);
+        for (int i = 0; i < Q; ++i) s[4 + i] = f[i] - feq_dev(i, d, rho, vx, vy, vz, usq);
     }
     // the grid has whole waves and no lane has left: every lane is active
 #pragma unroll
     for (int j = 0; j < 4 + Q; ++j) s[j] = group8_sum(s[j]);
     if (!on) return;
     const R fac = (R)f2c_factor(Lc.inst[Lc.binst[g >> 6]].tau0, Lc.level);
-    const R r = s[0] * (R)0.125;
+    const R dm = s[0] * (R)0.125;
+    const R r = (R)1 + dm;
     const R ux = (s[1] * (R)0.125) / r, uy = (s[2] * (R)0.125) / r, uz = (s[3] * (R)0.125) / r;
     const R usq = ux * ux + uy * uy + uz * uz;
+    const uint32_t rb = slot_bits(Lc, g);
 #pragma unroll
     for (int i = 0; i < Q; ++i)
-        if ((i & 7) == q) Store<T>::store(Ac, pidx(i, Lc.S, g), i, feq(i, r, ux, uy, uz, usq) + fac * (s[4 + i] * (R)0.125));
+        if ((i & 7) == q)
+            store_ghost<T>(Ac, pidx(i, Lc.S, g), i, feq_dev(i, dm, r, ux, uy, uz, usq) + fac * (s[4 + i] * (R)0.125),
+                           rb);
 }
 
 template <typename T>
@@ -455,27 +512,46 @@ __global__ __launch_bounds__(kWG) void k_brick_sums(LevelDev L, const T* __restr
     }
 }
 
+// Velocity of an active cell (FP64 sums over the stored populations).
+template <typename T>
+__device__ inline void cell_velocity(const LevelDev& L, const T* __restrict__ A, size_t s, float* u) {
+    double r = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const double f = (double)Store<T>::load(A, pidx(i, L.S, s), i);
+        r += f;
+        jx += cx(i) * f;
+        jy += cy(i) * f;
+        jz += cz(i) * f;
+    }
+    u[0] = (float)(jx / r);
+    u[1] = (float)(jy / r);
+    u[2] = (float)(jz / r);
+}
+
 // One wave per active brick: max over its active cells of |u - u_prev|
-// (infinity norm), u_prev replaced by u.
+// (infinity norm), u_prev replaced by u.  With inv_m > 0, u is the window
+// mean uacc * inv_m of the velocities sampled since the last call (and the
+// accumulator is cleared for the next window); with inv_m = 0, u is the
+// current velocity.
 template <typename T>
 __global__ __launch_bounds__(kWG) void k_brick_du(LevelDev L, const T* __restrict__ A, float* __restrict__ uprev,
-                                                  double* __restrict__ out) {
+                                                  float* __restrict__ uacc, float inv_m, double* __restrict__ out) {
     const int b = (int)(blockIdx.x * (kWG / 64) + (threadIdx.x >> 6));
     if (b >= L.nactive) return;
     const size_t s = (size_t)b * 64 + (threadIdx.x & 63);
     const bool act = (L.bflags[b] & 1u) != 0 || (L.cflags[s] & kActiveBit);
     double d = 0;
     if (act) {
-        double r = 0, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            const double f = (double)Store<T>::load(A, pidx(i, L.S, s), i);
-            r += f;
-            jx += cx(i) * f;
-            jy += cy(i) * f;
-            jz += cz(i) * f;
+        float u[3];
+        if (inv_m > 0.0f) {
+            for (int c = 0; c < 3; ++c) {
+                u[c] = uacc[s * 3 + c] * inv_m;
+                uacc[s * 3 + c] = 0.0f;
+            }
+        } else {
+            cell_velocity<T>(L, A, s, u);
         }
-        const float u[3] = {(float)(jx / r), (float)(jy / r), (float)(jz / r)};
         for (int c = 0; c < 3; ++c) {
             d = fmax(d, fabs((double)u[c] - (double)uprev[s * 3 + c]));
             uprev[s * 3 + c] = u[c];
@@ -483,6 +559,19 @@ __global__ __launch_bounds__(kWG) void k_brick_du(LevelDev L, const T* __restric
     }
     d = wave_max(d);
     if ((threadIdx.x & 63) == 0) out[b] = d;
+}
+
+// uacc += u at every active cell (one velocity sample of the current window).
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_velocity_sample(LevelDev L, const T* __restrict__ A,
+                                                         float* __restrict__ uacc) {
+    const int b = (int)(blockIdx.x * (kWG / 64) + (threadIdx.x >> 6));
+    if (b >= L.nactive) return;
+    const size_t s = (size_t)b * 64 + (threadIdx.x & 63);
+    if (!((L.bflags[b] & 1u) != 0 || (L.cflags[s] & kActiveBit))) return;
+    float u[3];
+    cell_velocity<T>(L, A, s, u);
+    for (int c = 0; c < 3; ++c) uacc[s * 3 + c] += u[c];
 }
 
 // One workgroup per instance: reduce the per-brick (or per-link) values of
@@ -866,9 +955,28 @@ hipError_t launch_brick_sums(const LevelDev& L, const T* A, double vol, double* 
 }
 
 template <typename T>
-hipError_t launch_brick_du(const LevelDev& L, const T* A, float* uprev, double* out, hipStream_t s) {
+hipError_t launch_brick_du(const LevelDev& L, const T* A, float* uprev, float* uacc, float inv_m, double* out,
+                           hipStream_t s) {
     if (L.nactive == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_brick_du<T>, dim3((L.nactive + 3) / 4), dim3(kWG), 0, s, L, A, uprev, out);
+    hipLaunchKernelGGL(k_brick_du<T>, dim3((L.nactive + 3) / 4), dim3(kWG), 0, s, L, A, uprev, uacc, inv_m, out);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_velocity_sample(const LevelDev& L, const T* A, float* uacc, hipStream_t s) {
+    if (L.nactive == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_velocity_sample<T>, dim3((L.nactive + 3) / 4), dim3(kWG), 0, s, L, A, uacc);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kWG) void k_fill_f32(float* __restrict__ p, size_t n, float v) {
+    for (size_t i = (size_t)blockIdx.x * kWG + threadIdx.x; i < n; i += (size_t)gridDim.x * kWG) p[i] = v;
+}
+
+hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const size_t g = (n + kWG - 1) / kWG;
+    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)(g > 65536 ? 65536 : g)), dim3(kWG), 0, s, p, n, v);
     return hipGetLastError();
 }
 
@@ -948,7 +1056,8 @@ hipError_t launch_ghost_eq(const uint32_t* c2f_gchild, const uint32_t* c2f_gcap,
                                            const LevelDev&, const T*, T*, hipStream_t);                         \
     template hipError_t launch_init<T>(const LevelDev&, T*, T*, hipStream_t);                                   \
     template hipError_t launch_brick_sums<T>(const LevelDev&, const T*, double, double*, hipStream_t);          \
-    template hipError_t launch_brick_du<T>(const LevelDev&, const T*, float*, double*, hipStream_t);            \
+    template hipError_t launch_brick_du<T>(const LevelDev&, const T*, float*, float*, float, double*, hipStream_t); \
+    template hipError_t launch_velocity_sample<T>(const LevelDev&, const T*, float*, hipStream_t);               \
     template hipError_t launch_link_force<T>(const uint32_t*, uint32_t, size_t, const T*, double, double*,      \
                                              hipStream_t);                                                      \
     template hipError_t launch_extract<T>(const T*, size_t, const int64_t*, size_t, double*, hipStream_t);     \

@@ -35,7 +35,11 @@ hipError_t launch_init(const LevelDev& L, T* A, T* B, hipStream_t s);
 template <typename T>
 hipError_t launch_brick_sums(const LevelDev& L, const T* A, double vol, double* out, hipStream_t s);
 template <typename T>
-hipError_t launch_brick_du(const LevelDev& L, const T* A, float* uprev, double* out, hipStream_t s);
+hipError_t launch_brick_du(const LevelDev& L, const T* A, float* uprev, float* uacc, float inv_m, double* out,
+                           hipStream_t s);
+template <typename T>
+hipError_t launch_velocity_sample(const LevelDev& L, const T* A, float* uacc, hipStream_t s);
+hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
 hipError_t launch_inst_reduce(const double* vals, int ncomp, const uint32_t* beg, int ninst, int op, int first,
                               double* acc, hipStream_t s);
 template <typename T>

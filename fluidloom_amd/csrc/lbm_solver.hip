@@ -202,6 +202,7 @@ struct DevLevel {
     double* itau_d = nullptr;
     double* bvals = nullptr;  // [max(nactive * 4, nlinks * 3)] reduction staging
     float* uprev = nullptr;   // [nactive * 64 * 3] velocity snapshot (allocated on first use)
+    float* uacc = nullptr;    // [nactive * 64 * 3] sum of the window's velocity samples (first use)
     void* buf[2] = {nullptr, nullptr};
     void* cap[2] = {nullptr, nullptr};
     int cur = 0;
@@ -240,6 +241,7 @@ struct fl_lbm_solver {
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     uint64_t graph_launches = 0;
+    int nsamples = 0;  // velocity samples in the current steady-state window
 
     fl_lbm_solver() {
         for (int l = 0; l < 24; ++l) variant[l] = 4;  // bpw2-xcd until tuned
@@ -1047,10 +1049,37 @@ fl_status force(fl_lbm_solver* S, double* out3) {
     return FL_OK;
 }
 
+// The window's velocity accumulator of a level, allocated and cleared on
+// first use (a fill kernel, not a memset).
+fl_status level_uacc(fl_lbm_solver* S, DevLevel& D) {
+    if (D.uacc) return FL_OK;
+    void* a;
+    const size_t n = (size_t)D.ld.nactive * 64 * 3;
+    fl_status r = dev_alloc(S, &a, n * sizeof(float));
+    if (r) return r;
+    D.uacc = static_cast<float*>(a);
+    FL_HIP(launch_fill_f32(D.uacc, n, 0.0f, S->stream));
+    return FL_OK;
+}
+
+template <typename T>
+fl_status velocity_sample(fl_lbm_solver* S) {
+    for (int l = 0; l < S->L; ++l) {
+        DevLevel& D = S->d[l];
+        if (D.ld.nactive == 0) continue;
+        fl_status r = level_uacc(S, D);
+        if (r) return r;
+        FL_HIP(launch_velocity_sample<T>(D.ld, static_cast<const T*>(D.buf[D.cur]), D.uacc, S->stream));
+    }
+    ++S->nsamples;
+    return FL_OK;
+}
+
 template <typename T>
 fl_status velocity_change(fl_lbm_solver* S, double* du) {
     const int NI = S->ninst();
     bool fresh = false, first = true;
+    const float inv_m = S->nsamples > 0 ? 1.0f / (float)S->nsamples : 0.0f;
     for (int l = 0; l < S->L; ++l) {
         DevLevel& D = S->d[l];
         if (D.ld.nactive == 0) continue;
@@ -1061,10 +1090,16 @@ fl_status velocity_change(fl_lbm_solver* S, double* du) {
             D.uprev = static_cast<float*>(a);
             fresh = true;
         }
-        FL_HIP(launch_brick_du<T>(D.ld, static_cast<const T*>(D.buf[D.cur]), D.uprev, D.bvals, S->stream));
+        if (inv_m > 0.0f) {
+            fl_status r = level_uacc(S, D);
+            if (r) return r;
+        }
+        FL_HIP(launch_brick_du<T>(D.ld, static_cast<const T*>(D.buf[D.cur]), D.uprev, D.uacc, inv_m, D.bvals,
+                                  S->stream));
         FL_HIP(launch_inst_reduce(D.bvals, 1, D.abeg, NI, 1, first ? 1 : 0, S->acc, S->stream));
         first = false;
     }
+    S->nsamples = 0;
     if (first) {
         for (int k = 0; k < NI; ++k) du[k] = 0.0;
         return FL_OK;
@@ -1581,6 +1616,16 @@ fl_status fl_lbm_force(fl_lbm_solver* S, double* F) {
     return FL_OK;
 }
 
+fl_status fl_lbm_velocity_sample(fl_lbm_solver* S) {
+    FL_REQUIRE(S, "null solver");
+    switch (S->storage) {
+        case FL_STORE_F32: return velocity_sample<float>(S);
+        case FL_STORE_F16: return velocity_sample<_Float16>(S);
+        case FL_STORE_F16C: return velocity_sample<fp16c>(S);
+        default: return velocity_sample<double>(S);
+    }
+}
+
 fl_status fl_lbm_velocity_change(fl_lbm_solver* S, double* du) {
     FL_REQUIRE(S && du, "null pointer");
     switch (S->storage) {
@@ -1747,6 +1792,8 @@ fl_status fl_lbm_regrid_batch(fl_lbm_solver* S, const size_t* offsets, const int
     std::swap(S->acc, N->acc);
     std::swap(S->allocs, N->allocs);
     std::swap(S->device_bytes, N->device_bytes);
+    // the velocity snapshots and window sums belonged to the old layout
+    S->nsamples = 0;
     return FL_OK;
 }
 

@@ -155,6 +155,9 @@ class LbmBatch:
         check(lib().fl_lbm_velocity_change(self.h, du.ctypes.data))
         return du
 
+    def velocity_sample(self):
+        check(lib().fl_lbm_velocity_sample(self.h))
+
     def set_live(self, inst, live):
         check(lib().fl_lbm_set_live(self.h, int(inst), int(bool(live))))
 

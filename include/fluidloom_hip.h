@@ -363,8 +363,16 @@ fl_status fl_lbm_totals_batch(fl_lbm_solver* solver, double* h_mass, double* h_m
 fl_status fl_lbm_force_batch(fl_lbm_solver* solver, double* h_force3n);
 /* Per instance, max over its fluid leaves of |u - u_snapshot| (infinity
  * norm over components), then snapshot = u.  The first call takes the
- * snapshot and reports HUGE_VAL.  Steady-state test of the serving loop. */
+ * snapshot and reports HUGE_VAL.  Steady-state test of the serving loop.
+ * When fl_lbm_velocity_sample was called since the last call, u is the mean
+ * of those samples (a window mean, which averages out the per-step noise of
+ * 16-bit storage's stochastic rounding) and the sum restarts; otherwise u
+ * is the current velocity.  No counterpart in the reference (it has no
+ * lattice solver). */
 fl_status fl_lbm_velocity_change(fl_lbm_solver* solver, double* h_du);
+/* Adds the current velocity of every fluid leaf (FP32) to the window sum
+ * that the next fl_lbm_velocity_change averages.  Stream-ordered, no sync. */
+fl_status fl_lbm_velocity_sample(fl_lbm_solver* solver);
 /* A non-live instance is no longer stepped (its slots keep their values);
  * the serving loop retires finished requests this way. */
 fl_status fl_lbm_set_live(fl_lbm_solver* solver, int inst, int live);
