@@ -93,3 +93,38 @@ def test_plain_server_one_at_a_time_same_results(fl):
         assert d.steps == batched[d.rid].steps
         assert np.array_equal(d.result["f"], batched[d.rid].result["f"]), d.rid
 
+
+
+def test_served_request_matches_oracle(fl, oracle):
+    """Requests served inside a mix (a refined cavity and a refined sphere,
+    batched with two other cavities) against the FP64 oracle run of each on
+    its own mesh for the same step count: velocity relative L2 within 1e-4
+    in FP32 storage and 1e-3 in FP16 storage."""
+    from fluidloom_amd.serving import Server
+    from tests.lbm_cases import OUT, WALL
+    cv, _, _ = oracle.lattice()
+    for storage, tol in (("f32", 1e-4), ("f16", 1e-3)):
+        mix = Server(storage=storage, check_every=20, tol=0.0, max_steps=200, keep_results=True)
+        done = {r.rid: r for r in mix.run(small_requests())}
+        for rid in (0, 1):
+            d = done[rid]
+            s, m = d.spec, d.mesh
+            if d.kind == "cavity":
+                n = (s["n0"],) * 3
+                bcu = [(0.0, 0.0, 0.0)] * 5 + [(s["lid"], 0.0, 0.0)]
+                o = oracle.Lbm(*n, s["max_level"], s["tau"], [WALL] * 6, bcu, m["x"], m["y"], m["z"], m["level"],
+                               m["state"])
+            else:
+                n = (s["nx"], s["ny"], s["nz"])
+                tau0 = 3 * s["u_in"] * 2 * s["r"] / s["re"] + 0.5
+                bcu = [(s["u_in"], 0.0, 0.0)] + [(0.0, 0.0, 0.0)] * 5
+                o = oracle.Lbm(*n, s["max_level"], tau0, [WALL, OUT, WALL, WALL, WALL, WALL], bcu, m["x"], m["y"],
+                               m["z"], m["level"], m["state"], u_init=(s["u_in"], 0.0, 0.0))
+            o.step(d.steps)
+            o.fx, o.fy, o.fz, o.flv = (np.ascontiguousarray(a) for a in d.result["fluid"])
+            _, uo = o.moments()
+            f = d.result["f"]
+            u = (f @ cv) / f.sum(1)[:, None]
+            err = np.linalg.norm(u - uo) / np.linalg.norm(uo)
+            print(f"served {d.kind} ({storage}, {d.steps} steps) vs FP64 oracle: velocity rel L2 {err:.2e}", flush=True)
+            assert d.steps == 200 and err < tol, (storage, rid, err)
