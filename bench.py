@@ -5,14 +5,24 @@ Workload (the single-GPU configuration of the metric): base grid 256x128x128,
 sphere of radius 16 cells centred at (64,64,64), 3 refinement levels within
 2 radii of the centre (D3Q19 neighbour-cube 2:1 balance), velocity inlet
 u = 0.05 on -x, outflow on +x, bounce-back walls elsewhere, tau0 for Re = 100
-on the diameter, regrid check every 100 steps.  A "step" is one coarse
+on the diameter, a device regrid pass every 100 steps.  A "step" is one coarse
 (level-0) time step with all level sub-cycles; MLUPS counts every leaf
 update (level l leaves update 2^l times per step).
 
+Every regrid_every steps the timed loop runs a full device regrid pass over
+the leaves (refinement marks, then fl_regrid: D3Q19 balance, merge,
+compaction, split).  The sphere's mesh is static, so the pass returns the
+same leaves; the bench checks that it split and merged nothing, which is
+what lets it keep the solver's layout.  One rebuild of the solver's layout
+for the same leaves (fl_lbm_regrid) is timed beside the loop.
+
 Prints one JSON line.  `value` is whole-job MLUPS; `roofline` is the level-3
 collide-stream kernel (the dominant kernel) measured with HIP events on the
-solver's stream; `cpu_baseline` is the FP64 oracle (OpenMP) on a bounded,
-64x smaller instance of the same configuration.
+solver's stream, with `traffic` (HBM bytes per launch) taken from the
+committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this command;
+`cpu_baseline` is the CPU port of the lattice step in FP32 (the oracle's
+FP32 build, OpenMP) on a bounded quarter-size instance of the same
+configuration.
 """
 from __future__ import annotations
 
@@ -46,19 +56,22 @@ def sphere_params(nx0, ny0, nz0, r, u_in, re):
 
 
 def cpu_baseline(budget_s=15.0):
-    """FP64 oracle on a 1/64-size sphere instance (same shape of workload)."""
+    """The lattice step on the host in FP32 (the oracle's FP32 build, all
+    host cores) on the quarter-size sphere: base 64x32x32, r = 4, 3 levels
+    within 2r, Re 100 -- the same workload at 1/64 of the leaves.  Its leaves
+    come from the device mesh builder; only the stepping is timed."""
+    from fluidloom_amd import meshes
     from oracle import oracle as O
-    from tests.amr_util import ball_region, mark_solid_ball, refine_mesh, uniform_cells
     nx0, ny0, nz0, r = 64, 32, 32, 4.0
     ctr = (16.0, 16.0, 16.0)
-    base = uniform_cells(O, (nx0, ny0, nz0))
-    c = refine_mesh(O, base, (nx0, ny0, nz0), [(3, ball_region(ctr, 2 * r))], 3)
-    c = mark_solid_ball(c, ctr, r)
+    m = meshes.sphere(nx0, ny0, nz0, ctr, r, 2 * r, 3)
     tau0, bc, bcu = sphere_params(nx0, ny0, nz0, r, 0.05, 100.0)
     bci = [{"wall": 0, "outflow": 1}[b] for b in bc]
-    s = O.Lbm(nx0, ny0, nz0, 3, tau0, bci, bcu, c.x, c.y, c.z, c.level, c.state, u_init=(0.05, 0, 0))
-    fluid = c.state == 0
-    upd = int(sum(int((fluid & (c.level == l)).sum()) << l for l in range(4)))
+    s = O.Lbm(nx0, ny0, nz0, 3, tau0, bci, bcu, m["x"], m["y"], m["z"], m["level"], m["state"],
+              u_init=(0.05, 0, 0), single=True)
+    fluid = np.asarray(m["state"]) == 0
+    lv = np.asarray(m["level"])
+    upd = int(sum(int((fluid & (lv == l)).sum()) << l for l in range(4)))
     s.step(1)
     t0 = time.perf_counter()
     steps = 0
@@ -68,8 +81,8 @@ def cpu_baseline(budget_s=15.0):
     dt = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     return {"value": round(upd * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": cores, "kind": "port",
-            "sample": f"FP64 oracle, sphere {nx0}x{ny0}x{nz0} r={r:g} 3 levels ({upd} leaf updates/step), "
-                      f"{steps} steps in {dt:.1f} s"}
+            "sample": f"FP32 CPU port of the lattice step (oracle FP32 build, OpenMP), sphere "
+                      f"{nx0}x{ny0}x{nz0} r={r:g} 3 levels ({upd} leaf updates/step), {steps} steps in {dt:.1f} s"}
 
 
 def copy_bandwidth(_lib, L, stream, nbytes=2 << 30, iters=10):
@@ -142,11 +155,37 @@ def main():
         log(f"autotuned collide variants per level: {[variant_names()[v] for v in chosen]}")
     if not args.no_graph:
         solver.set_graph(True)
-    # leaves on the device for the periodic regrid check
+    # leaves on the device for the periodic regrid pass
     cells = amr.GpuCells.from_host(**{k: mesh[k] for k in ("x", "y", "z", "level", "state")})
+    # the pass's output and workspace, allocated once
+    regrid_out = amr.GpuCells(cells.n + cells.n // 8 + 64)
+    regrid_tmp = amr.regrid_temp(cells.n)
 
-    def regrid_check():
-        return amr.mark_ball(cells, ctr, 2 * r, levels)
+    def regrid_pass():
+        """Refinement marks + one device regrid pass; the sphere's mesh is
+        static, so the pass must return the leaves unchanged."""
+        marked = amr.mark_ball(cells, ctr, 2 * r, levels)
+        if marked:
+            raise RuntimeError("the sphere's mesh changed at a regrid; a static mesh is expected")
+        new, rst = amr.regrid(cells, balance_mode=1, max_split_level=levels, root=(nx0, ny0, nz0),
+                              merge_balance_check=1, out=regrid_out, temp=regrid_tmp)
+        same = rst.num_split == 0 and rst.num_groups == 0 and new.n == cells.n
+        if not same:
+            raise RuntimeError("the sphere's mesh changed at a regrid; a static mesh is expected")
+
+    # untimed: one regrid pass alone, and one rebuild of the solver's layout
+    # for the same leaves (what a regrid that changes the mesh costs)
+    _lib.check(L.fl_device_sync())
+    t_r0 = time.perf_counter()
+    regrid_pass()
+    _lib.check(L.fl_device_sync())
+    regrid_pass_s = time.perf_counter() - t_r0
+    t_r0 = time.perf_counter()
+    solver.regrid(mesh["x"], mesh["y"], mesh["z"], mesh["level"], mesh["state"])
+    solver.sync()
+    layout_rebuild_s = time.perf_counter() - t_r0
+    log(f"regrid pass {regrid_pass_s * 1e3:.1f} ms; solver layout rebuild for the same leaves "
+        f"{layout_rebuild_s:.2f} s")
 
     solver.step(args.warmup)
     if not args.no_graph:
@@ -168,10 +207,9 @@ def main():
             n = min(n, args.regrid_every - k % args.regrid_every)
         solver.step(n)
         k += n
-        # the regrid check after every regrid_every-th step, inside the timed region
+        # the regrid pass after every regrid_every-th step, inside the timed region
         if args.regrid_every and k % args.regrid_every == 0:
-            if regrid_check() != 0:
-                raise RuntimeError("sphere refinement changed; static mesh expected")
+            regrid_pass()
             regrids += 1
         if k % 50 == 0:
             log(f"step {k}/{args.steps}")
@@ -233,18 +271,23 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"flow past a sphere: base {nx0}x{ny0}x{nz0}, r={r:g} at {ctr}, "
                                f"{levels} levels within {2 * r:g} cells, Re={re:g}, u_in={u_in}, "
-                               f"tau0={tau0:.4f}, regrid check every {args.regrid_every} steps",
+                               f"tau0={tau0:.4f}, device regrid pass every {args.regrid_every} steps",
                    "leaves_per_level": leaves, "updates_per_step": upd, "storage": args.storage,
                    "parallelism": "single GPU"},
         "hbm_gbps_algorithmic": round(mlups * 1e6 * bpu / 1e9, 1),
         "gpu_event_ms": round(ms.value, 3),
-        "regrid_checks": regrids,
+        "regrid_passes": regrids,
+        "regrid_pass_ms": round(regrid_pass_s * 1e3, 2),
+        "layout_rebuild_s": round(layout_rebuild_s, 3),
         "collide_variants": [variant_names()[int(st2.variant[l])] for l in range(levels + 1)],
         "graph": bool(st2.graph),
         "drag_coefficient": round(float(cd), 4),
         "mass": mass,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": (f"profiles/{tf.name}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of "
+                                        f"this command, scaled by the copy-kernel calibration")
+                                       if traffic is not None else None,
                      "kernel": f"k_collide {finest_variant} level {levels} ({leaves[levels]} leaves x {bpu} B)"},
         "copy_kernel_gbs": round(copy_gbs, 1),
         "finest_variant_ms": variant_ms,

@@ -95,15 +95,25 @@ class RegridStats:
     balance_converged: bool
 
 
+def regrid_temp(n) -> DeviceArray:
+    """A workspace for regrid() of up to n cells (reusable across calls)."""
+    return DeviceArray(max(lib().fl_regrid_temp_bytes(n), 1), np.uint8)
+
+
 def regrid(cells: GpuCells, balance_mode=1, anchor_level=10, max_split_level=8, max_iters=64,
            root=(1, 1, 1), merge_balance_check=0, fields=None, ncomp=0, avg_rule=0,
-           out_capacity=None, stream=None):
+           out_capacity=None, stream=None, out=None, temp=None):
     """One device regrid pass (balance, merge, compaction, split, parents).
 
-    Returns (new GpuCells, RegridStats[, new fields]).  Flags are consumed."""
+    `out` (GpuCells) and `temp` (from regrid_temp) may be passed in to reuse
+    their memory; otherwise they are allocated here.  Returns (new GpuCells,
+    RegridStats[, new fields]).  Flags are consumed."""
     n = cells.n
-    cap = out_capacity or (n + 7 * n if n else 1)
-    out = GpuCells(cap)
+    if out is None:
+        cap = out_capacity or (n + 7 * n if n else 1)
+        out = GpuCells(cap)
+    else:
+        cap = out.capacity
     cfg = RegridConfig()
     cfg.balance_mode, cfg.anchor_level, cfg.max_split_level = balance_mode, anchor_level, max_split_level
     cfg.max_balance_iters = max_iters
@@ -112,7 +122,7 @@ def regrid(cells: GpuCells, balance_mode=1, anchor_level=10, max_split_level=8, 
     cfg.merge_balance_check, cfg.ncomp, cfg.avg_rule = merge_balance_check, ncomp, avg_rule
     fout = DeviceArray(max(cap * max(ncomp, 1), 1), np.float32) if ncomp else None
     tb = lib().fl_regrid_temp_bytes(n)
-    tmp = DeviceArray(max(tb, 1), np.uint8)
+    tmp = temp if temp is not None and temp.nbytes >= tb else DeviceArray(max(tb, 1), np.uint8)
     res = RegridResult()
     cs, os_ = cells.struct(), out.struct()
     check(lib().fl_regrid(C.byref(cfg), C.byref(cs), n, C.byref(os_), cap,

@@ -27,13 +27,21 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Populations and per-cell arithmetic are `real`: double for the oracle;
+ * the Makefile also builds a float copy (liboracle_f32.so) that only the
+ * benchmark's CPU baseline times.  The API stays double either way. */
+#ifndef OR_REAL
+#define OR_REAL double
+#endif
+typedef OR_REAL real;
+
 #define Q 19
 enum { T_NONE = 0, T_FLUID = 1, T_SOLID = 2, T_C2F = 3, T_F2C = 4, T_WALL = 5 };
 
 static const int CX[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
 static const int CY[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
 static const int CZ[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
-static double W[Q];
+static real W[Q];
 static int OPP[Q];
 static int lattice_ready = 0;
 
@@ -61,10 +69,10 @@ typedef struct {
     size_t N;
     uint8_t* type;
     int64_t* c2f_src;   /* coarse position for T_C2F */
-    double* f[2];       /* [Q][N] */
-    double* cap[2];     /* [Q][N] at capture cells */
+    real* f[2];       /* [Q][N] */
+    real* cap[2];     /* [Q][N] at capture cells */
     int cur;
-    double tau;
+    real tau;
     size_t nfluid, nc2f, nf2c, ncap;
     int64_t* fluid;     /* positions */
     int64_t* c2f;
@@ -77,7 +85,7 @@ struct or_lbm {
     or_lbm_params p;
     int L;              /* number of levels */
     level_t lv[24];
-    double c2f_mult, f2c_mult, slope_mult; /* 1.0; tests plant mistakes through these */
+    real c2f_mult, f2c_mult, slope_mult; /* 1.0; tests plant mistakes through these */
 };
 
 static inline int64_t pidx(const level_t* v, int x, int y, int z) {
@@ -90,14 +98,14 @@ static inline void pxyz(const level_t* v, int64_t p, int* x, int* y, int* z) {
 }
 static inline int regular(int t) { return t == T_FLUID || t == T_C2F || t == T_F2C; }
 
-static inline double feq(double rho, double ux, double uy, double uz, int i) {
-    double cu = CX[i] * ux + CY[i] * uy + CZ[i] * uz;
-    double usq = ux * ux + uy * uy + uz * uz;
+static inline real feq(real rho, real ux, real uy, real uz, int i) {
+    real cu = CX[i] * ux + CY[i] * uy + CZ[i] * uz;
+    real usq = ux * ux + uy * uy + uz * uz;
     return W[i] * rho * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq);
 }
 
-static inline void moments(const double* f, double* rho, double* ux, double* uy, double* uz) {
-    double r = 0, jx = 0, jy = 0, jz = 0;
+static inline void moments(const real* f, real* rho, real* ux, real* uy, real* uz) {
+    real r = 0, jx = 0, jy = 0, jz = 0;
     for (int i = 0; i < Q; ++i) {
         r += f[i];
         jx += CX[i] * f[i]; jy += CY[i] * f[i]; jz += CZ[i] * f[i];
@@ -155,7 +163,7 @@ static int64_t coarse_nb(const or_lbm* s, const level_t* c, int64_t cp, int a, i
     return c->type[np] == T_FLUID ? np : -1;
 }
 
-static void gather(const or_lbm* s, const level_t* v, int64_t pos, const double* buf, double* out) {
+static void gather(const or_lbm* s, const level_t* v, int64_t pos, const real* buf, real* out) {
     int x, y, z;
     pxyz(v, pos, &x, &y, &z);
     const size_t N = v->N;
@@ -174,8 +182,8 @@ static void gather(const or_lbm* s, const level_t* v, int64_t pos, const double*
     }
 }
 
-static void collide(const double* f, double tau, double* o) {
-    double rho, ux, uy, uz;
+static void collide(const real* f, real tau, real* o) {
+    real rho, ux, uy, uz;
     moments(f, &rho, &ux, &uy, &uz);
     for (int i = 0; i < Q; ++i) o[i] = f[i] - (f[i] - feq(rho, ux, uy, uz, i)) / tau;
 }
@@ -293,13 +301,13 @@ or_lbm* or_lbm_create(const or_lbm_params* p, const int32_t* x, const int32_t* y
             if (v->is_cap[q]) v->capl[v->ncap++] = (int64_t)q;
         }
         for (int b = 0; b < 2; ++b) {
-            v->f[b] = (double*)calloc((size_t)Q * v->N, sizeof(double));
-            v->cap[b] = (double*)calloc((size_t)Q * v->N, sizeof(double));
+            v->f[b] = (real*)calloc((size_t)Q * v->N, sizeof(real));
+            v->cap[b] = (real*)calloc((size_t)Q * v->N, sizeof(real));
         }
         for (size_t q = 0; q < v->N; ++q) {
             if (!regular(v->type[q])) continue;
             for (int i = 0; i < Q; ++i) {
-                double e = feq(rho_init, u_init[0], u_init[1], u_init[2], i);
+                real e = feq(rho_init, u_init[0], u_init[1], u_init[2], i);
                 v->f[0][(size_t)i * v->N + q] = e;
                 v->f[1][(size_t)i * v->N + q] = e;
             }
@@ -323,26 +331,26 @@ void or_lbm_destroy(or_lbm* s) {
 static void fill_f2c(or_lbm* s, int l) {
     level_t* v = &s->lv[l];
     level_t* f = &s->lv[l + 1];
-    const double* fprev = f->f[1 - f->cur];
-    double* dst = v->f[v->cur];
-    const double fac = (1.0 - 1.0 / v->tau) * (2.0 * v->tau / f->tau) * s->f2c_mult;
+    const real* fprev = f->f[1 - f->cur];
+    real* dst = v->f[v->cur];
+    const real fac = (1.0 - 1.0 / v->tau) * (2.0 * v->tau / f->tau) * s->f2c_mult;
 #pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < (int64_t)v->nf2c; ++k) {
         int64_t g = v->f2c[k];
         int x, y, z;
         pxyz(v, g, &x, &y, &z);
-        double sr = 0, sjx = 0, sjy = 0, sjz = 0, sneq[Q];
+        real sr = 0, sjx = 0, sjy = 0, sjz = 0, sneq[Q];
         for (int i = 0; i < Q; ++i) sneq[i] = 0;
         for (int q = 0; q < 8; ++q) {
             int64_t cp = pidx(f, 2 * x + (q & 1), 2 * y + ((q >> 1) & 1), 2 * z + ((q >> 2) & 1));
-            double fk[Q], r, ux, uy, uz;
+            real fk[Q], r, ux, uy, uz;
             gather(s, f, cp, fprev, fk);
             moments(fk, &r, &ux, &uy, &uz);
             sr += r; sjx += r * ux; sjy += r * uy; sjz += r * uz;
             for (int i = 0; i < Q; ++i) sneq[i] += fk[i] - feq(r, ux, uy, uz, i);
         }
-        double r = sr * 0.125;
-        double ux = (sjx * 0.125) / r, uy = (sjy * 0.125) / r, uz = (sjz * 0.125) / r;
+        real r = sr * 0.125;
+        real ux = (sjx * 0.125) / r, uy = (sjy * 0.125) / r, uz = (sjz * 0.125) / r;
         for (int i = 0; i < Q; ++i)
             dst[(size_t)i * v->N + g] = feq(r, ux, uy, uz, i) + fac * (sneq[i] * 0.125);
     }
@@ -350,17 +358,17 @@ static void fill_f2c(or_lbm* s, int l) {
 
 static void capture(or_lbm* s, int l, int slot) {
     level_t* v = &s->lv[l];
-    const double* prev = v->f[1 - v->cur];
+    const real* prev = v->f[1 - v->cur];
 #pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < (int64_t)v->ncap; ++k) {
         int64_t c = v->capl[k];
-        double fk[Q];
+        real fk[Q];
         gather(s, v, c, prev, fk);
         for (int i = 0; i < Q; ++i) v->cap[slot][(size_t)i * v->N + c] = fk[i];
     }
 }
 
-static void cap_moments(const level_t* c, int64_t cp, double alpha, double* f, double* m) {
+static void cap_moments(const level_t* c, int64_t cp, real alpha, real* f, real* m) {
     for (int i = 0; i < Q; ++i)
         f[i] = (1.0 - alpha) * c->cap[0][(size_t)i * c->N + cp] + alpha * c->cap[1][(size_t)i * c->N + cp];
     moments(f, &m[0], &m[1], &m[2], &m[3]);
@@ -372,27 +380,27 @@ static void cap_moments(const level_t* c, int64_t cp, double alpha, double* f, d
  * one-sided where only one side is a fluid leaf, or none); the
  * non-equilibrium part is P's, rescaled:
  *   f* = feq(rho_g, u_g) + (1 - 1/tau_f)(tau_f / (2 tau_c)) (f_P - feq(rho_P, u_P)) */
-static void fill_c2f(or_lbm* s, int l, double alpha) {
+static void fill_c2f(or_lbm* s, int l, real alpha) {
     level_t* v = &s->lv[l];
     level_t* c = &s->lv[l - 1];
-    double* dst = v->f[v->cur];
-    const double fac = (1.0 - 1.0 / v->tau) * (v->tau / (2.0 * c->tau)) * s->c2f_mult;
+    real* dst = v->f[v->cur];
+    const real fac = (1.0 - 1.0 / v->tau) * (v->tau / (2.0 * c->tau)) * s->c2f_mult;
 #pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < (int64_t)v->nc2f; ++k) {
         int64_t g = v->c2f[k];
         int64_t cp = v->c2f_src[g];
         int gc[3];
         pxyz(v, g, &gc[0], &gc[1], &gc[2]);
-        double fp[Q], mp[4], fn[Q], mm[4], mq[4], phi[4];
+        real fp[Q], mp[4], fn[Q], mm[4], mq[4], phi[4];
         cap_moments(c, cp, alpha, fp, mp);
         for (int j = 0; j < 4; ++j) phi[j] = mp[j];
         for (int a = 0; a < 3; ++a) {
             const int64_t nm = coarse_nb(s, c, cp, a, -1), nq = coarse_nb(s, c, cp, a, +1);
             if (nm >= 0) cap_moments(c, nm, alpha, fn, mm);
             if (nq >= 0) cap_moments(c, nq, alpha, fn, mq);
-            const double dl = ((gc[a] & 1) ? 0.25 : -0.25) * s->slope_mult;
+            const real dl = ((gc[a] & 1) ? 0.25 : -0.25) * s->slope_mult;
             for (int j = 0; j < 4; ++j) {
-                double gr = 0.0;
+                real gr = 0.0;
                 if (nm >= 0 && nq >= 0) gr = 0.5 * (mq[j] - mm[j]);
                 else if (nq >= 0) gr = mq[j] - mp[j];
                 else if (nm >= 0) gr = mp[j] - mm[j];
@@ -400,7 +408,7 @@ static void fill_c2f(or_lbm* s, int l, double alpha) {
             }
         }
         for (int i = 0; i < Q; ++i) {
-            const double e = feq(phi[0], phi[1], phi[2], phi[3], i);
+            const real e = feq(phi[0], phi[1], phi[2], phi[3], i);
             dst[(size_t)i * v->N + g] = e + fac * (fp[i] - feq(mp[0], mp[1], mp[2], mp[3], i));
         }
     }
@@ -408,12 +416,12 @@ static void fill_c2f(or_lbm* s, int l, double alpha) {
 
 static void collide_stream(or_lbm* s, int l) {
     level_t* v = &s->lv[l];
-    const double* src = v->f[v->cur];
-    double* dst = v->f[1 - v->cur];
+    const real* src = v->f[v->cur];
+    real* dst = v->f[1 - v->cur];
 #pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < (int64_t)v->nfluid; ++k) {
         int64_t p = v->fluid[k];
-        double fk[Q], o[Q];
+        real fk[Q], o[Q];
         gather(s, v, p, src, fk);
         collide(fk, v->tau, o);
         for (int i = 0; i < Q; ++i) dst[(size_t)i * v->N + p] = o[i];
@@ -477,9 +485,11 @@ void or_lbm_moments(const or_lbm* s, const int32_t* x, const int32_t* y, const i
         int64_t p = leaf_pos(s, x[k], y[k], z[k], level[k]);
         if (p < 0) { rho[k] = NAN; u[3 * k] = u[3 * k + 1] = u[3 * k + 2] = NAN; continue; }
         const level_t* v = &s->lv[level[k]];
-        double f[Q];
+        real f[Q];
         for (int i = 0; i < Q; ++i) f[i] = v->f[v->cur][(size_t)i * v->N + p];
-        moments(f, &rho[k], &u[3 * k], &u[3 * k + 1], &u[3 * k + 2]);
+        real r, ux, uy, uz;
+        moments(f, &r, &ux, &uy, &uz);
+        rho[k] = r; u[3 * k] = ux; u[3 * k + 1] = uy; u[3 * k + 2] = uz;
     }
 }
 
@@ -490,7 +500,7 @@ void or_lbm_totals(const or_lbm* s, double* mass, double mom[3]) {
         double vol = ldexp(1.0, -3 * l);
         for (size_t k = 0; k < v->nfluid; ++k) {
             int64_t p = v->fluid[k];
-            double f[Q], r, ux, uy, uz;
+            real f[Q], r, ux, uy, uz;
             for (int i = 0; i < Q; ++i) f[i] = v->f[v->cur][(size_t)i * v->N + p];
             moments(f, &r, &ux, &uy, &uz);
             *mass += r * vol;
@@ -503,8 +513,8 @@ void or_lbm_force(const or_lbm* s, double F[3]) {
     F[0] = F[1] = F[2] = 0;
     for (int l = 0; l < s->L; ++l) {
         const level_t* v = &s->lv[l];
-        double fx = 0, fy = 0, fz = 0;
-        const double* b = v->f[v->cur];
+        real fx = 0, fy = 0, fz = 0;
+        const real* b = v->f[v->cur];
         for (size_t k = 0; k < v->nfluid; ++k) {
             int64_t p = v->fluid[k];
             int x, y, z;
@@ -514,7 +524,7 @@ void or_lbm_force(const or_lbm* s, double F[3]) {
                 if (source(s, v, x, y, z, i, &sx, &sy, &sz, &face)) continue;
                 int t = v->type[pidx(v, sx, sy, sz)];
                 if (t != T_SOLID && t != T_WALL) continue;
-                double fo = b[(size_t)OPP[i] * v->N + p];
+                real fo = b[(size_t)OPP[i] * v->N + p];
                 fx += -2.0 * CX[i] * fo; fy += -2.0 * CY[i] * fo; fz += -2.0 * CZ[i] * fo;
             }
         }
@@ -533,7 +543,7 @@ void or_lbm_plant_coupling_error(or_lbm* s, double c2f_mult, double f2c_mult, do
  * Vorticity magnitude (level-0 units) at fluid leaves: central differences
  * over same-level fluid-leaf neighbours along each axis, one-sided when only
  * one side is a fluid leaf, zero when neither is; |w| * 2^level. */
-static int leaf_u(const or_lbm* s, int l, int x, int y, int z, double* u) {
+static int leaf_u(const or_lbm* s, int l, int x, int y, int z, real* u) {
     const level_t* v = &s->lv[l];
     int p[3] = {x, y, z};
     int e[3] = {v->X, v->Y, v->Z};
@@ -544,7 +554,7 @@ static int leaf_u(const or_lbm* s, int l, int x, int y, int z, double* u) {
     }
     int64_t q = pidx(v, p[0], p[1], p[2]);
     if (v->type[q] != T_FLUID) return 0;
-    double f[Q], r;
+    real f[Q], r;
     for (int i = 0; i < Q; ++i) f[i] = v->f[v->cur][(size_t)i * v->N + q];
     moments(f, &r, &u[0], &u[1], &u[2]);
     return 1;
@@ -554,12 +564,12 @@ void or_lbm_vorticity(const or_lbm* s, const int32_t* x, const int32_t* y, const
                       const uint8_t* level, size_t n, double* out) {
     for (size_t k = 0; k < n; ++k) {
         int l = level[k];
-        double u0[3], g[3][3];
+        real u0[3], g[3][3];
         if (l >= s->L || !leaf_u(s, l, x[k], y[k], z[k], u0)) { out[k] = 0; continue; }
         for (int a = 0; a < 3; ++a) {
             int dp[3] = {0, 0, 0};
             dp[a] = 1;
-            double up[3], um[3];
+            real up[3], um[3];
             int hp = leaf_u(s, l, x[k] + dp[0], y[k] + dp[1], z[k] + dp[2], up);
             int hm = leaf_u(s, l, x[k] - dp[0], y[k] - dp[1], z[k] - dp[2], um);
             for (int b = 0; b < 3; ++b) {
@@ -569,7 +579,7 @@ void or_lbm_vorticity(const or_lbm* s, const int32_t* x, const int32_t* y, const
                 else g[a][b] = 0.0;
             }
         }
-        double wx = g[1][2] - g[2][1], wy = g[2][0] - g[0][2], wz = g[0][1] - g[1][0];
+        real wx = g[1][2] - g[2][1], wy = g[2][0] - g[0][2], wz = g[0][1] - g[1][0];
         out[k] = sqrt(wx * wx + wy * wy + wz * wz) * ldexp(1.0, l);
     }
 }
@@ -593,18 +603,18 @@ or_lbm* or_lbm_regrid(const or_lbm* o, const int32_t* x, const int32_t* y, const
             int64_t p = v->fluid[k];
             int px, py, pz;
             pxyz(v, p, &px, &py, &pz);
-            double f[Q];
+            real f[Q];
             const level_t* ov = &o->lv[l];
             if (ov->type[p] == T_FLUID) {
                 for (int i = 0; i < Q; ++i) f[i] = ov->f[ov->cur][(size_t)i * ov->N + p];
             } else if (l > 0 && o->lv[l - 1].type[pidx(&o->lv[l - 1], px >> 1, py >> 1, pz >> 1)] == T_FLUID) {
                 const level_t* oc = &o->lv[l - 1];
-                double fp[Q], r, ux, uy, uz;
+                real fp[Q], r, ux, uy, uz;
                 gather(o, oc, pidx(oc, px >> 1, py >> 1, pz >> 1), oc->f[1 - oc->cur], fp);
                 moments(fp, &r, &ux, &uy, &uz);
-                double fac = (1.0 - 1.0 / v->tau) * (v->tau / (2.0 * oc->tau));
+                real fac = (1.0 - 1.0 / v->tau) * (v->tau / (2.0 * oc->tau));
                 for (int i = 0; i < Q; ++i) {
-                    double e = feq(r, ux, uy, uz, i);
+                    real e = feq(r, ux, uy, uz, i);
                     f[i] = e + fac * (fp[i] - e);
                 }
             } else {
@@ -615,19 +625,19 @@ or_lbm* or_lbm_regrid(const or_lbm* o, const int32_t* x, const int32_t* y, const
                 }
                 if (all) {
                     const level_t* of = &o->lv[l + 1];
-                    double sr = 0, sjx = 0, sjy = 0, sjz = 0, sneq[Q];
+                    real sr = 0, sjx = 0, sjy = 0, sjz = 0, sneq[Q];
                     for (int i = 0; i < Q; ++i) sneq[i] = 0;
                     for (int q = 0; q < 8; ++q) {
-                        double fk[Q], r, ux, uy, uz;
+                        real fk[Q], r, ux, uy, uz;
                         gather(o, of, pidx(of, 2 * px + (q & 1), 2 * py + ((q >> 1) & 1), 2 * pz + ((q >> 2) & 1)),
                                of->f[1 - of->cur], fk);
                         moments(fk, &r, &ux, &uy, &uz);
                         sr += r; sjx += r * ux; sjy += r * uy; sjz += r * uz;
                         for (int i = 0; i < Q; ++i) sneq[i] += fk[i] - feq(r, ux, uy, uz, i);
                     }
-                    double r = sr * 0.125;
-                    double ux = (sjx * 0.125) / r, uy = (sjy * 0.125) / r, uz = (sjz * 0.125) / r;
-                    double fac = (1.0 - 1.0 / v->tau) * (2.0 * v->tau / of->tau);
+                    real r = sr * 0.125;
+                    real ux = (sjx * 0.125) / r, uy = (sjy * 0.125) / r, uz = (sjz * 0.125) / r;
+                    real fac = (1.0 - 1.0 / v->tau) * (2.0 * v->tau / of->tau);
                     for (int i = 0; i < Q; ++i) f[i] = feq(r, ux, uy, uz, i) + fac * (sneq[i] * 0.125);
                 } else {
                     for (int i = 0; i < Q; ++i) f[i] = feq(1.0, 0, 0, 0, i);
@@ -647,11 +657,11 @@ or_lbm* or_lbm_regrid(const or_lbm* o, const int32_t* x, const int32_t* y, const
         for (size_t k = 0; k < v->nc2f; ++k) {
             int64_t g = v->c2f[k];
             const level_t* c = &s->lv[l - 1];
-            double f[Q], r, ux, uy, uz;
+            real f[Q], r, ux, uy, uz;
             for (int i = 0; i < Q; ++i) f[i] = c->f[0][(size_t)i * c->N + v->c2f_src[g]];
             moments(f, &r, &ux, &uy, &uz);
             for (int i = 0; i < Q; ++i) {
-                double e = feq(r, ux, uy, uz, i);
+                real e = feq(r, ux, uy, uz, i);
                 v->f[0][(size_t)i * v->N + g] = e;
                 v->f[1][(size_t)i * v->N + g] = e;
             }
@@ -661,18 +671,18 @@ or_lbm* or_lbm_regrid(const or_lbm* o, const int32_t* x, const int32_t* y, const
             const level_t* fv = &s->lv[l + 1];
             int px, py, pz;
             pxyz(v, g, &px, &py, &pz);
-            double sr = 0, sjx = 0, sjy = 0, sjz = 0;
+            real sr = 0, sjx = 0, sjy = 0, sjz = 0;
             for (int q = 0; q < 8; ++q) {
                 int64_t cp = pidx(fv, 2 * px + (q & 1), 2 * py + ((q >> 1) & 1), 2 * pz + ((q >> 2) & 1));
-                double f[Q], r, ux, uy, uz;
+                real f[Q], r, ux, uy, uz;
                 for (int i = 0; i < Q; ++i) f[i] = fv->f[0][(size_t)i * fv->N + cp];
                 moments(f, &r, &ux, &uy, &uz);
                 sr += r; sjx += r * ux; sjy += r * uy; sjz += r * uz;
             }
-            double r = sr * 0.125;
-            double ux = (sjx * 0.125) / r, uy = (sjy * 0.125) / r, uz = (sjz * 0.125) / r;
+            real r = sr * 0.125;
+            real ux = (sjx * 0.125) / r, uy = (sjy * 0.125) / r, uz = (sjz * 0.125) / r;
             for (int i = 0; i < Q; ++i) {
-                double e = feq(r, ux, uy, uz, i);
+                real e = feq(r, ux, uy, uz, i);
                 v->f[0][(size_t)i * v->N + g] = e;
                 v->f[1][(size_t)i * v->N + g] = e;
             }

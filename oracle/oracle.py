@@ -45,6 +45,7 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "_build" / "liboracle.so"
+LIB_F32 = HERE / "_build" / "liboracle_f32.so"
 REF = HERE / "_ref" / "libref_hilbert.so"
 
 vp = C.c_void_p
@@ -58,15 +59,32 @@ def build(ref: bool = False) -> None:
 
 
 _lib = None
+_lib_f32 = None
 _ref = None
 
 
 def lib():
+    """The FP64 oracle (the checker)."""
     global _lib
     if _lib is None:
-        if not LIB.exists():
+        _lib = _load(LIB)
+    return _lib
+
+
+def lib_f32():
+    """The same lattice step with FP32 populations and arithmetic: timed as
+    the benchmark's FP32 CPU baseline, never used as a checker."""
+    global _lib_f32
+    if _lib_f32 is None:
+        _lib_f32 = _load(LIB_F32)
+    return _lib_f32
+
+
+def _load(path):
+    if True:
+        if not path.exists():
             build()
-        L = C.CDLL(str(LIB))
+        L = C.CDLL(str(path))
         L.or_hilbert_encode.restype = C.c_uint64
         L.or_hilbert_encode.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int]
         L.or_key63.restype = C.c_uint64
@@ -116,8 +134,7 @@ def lib():
         L.or_lbm_vorticity.argtypes = [vp, vp, vp, vp, vp, sz, vp]
         L.or_lbm_regrid.restype = vp
         L.or_lbm_regrid.argtypes = [vp, vp, vp, vp, vp, vp, sz, C.POINTER(C.c_int)]
-        _lib = L
-    return _lib
+    return L
 
 
 def ref():
@@ -360,7 +377,8 @@ class Lbm:
     """FP64 oracle solver over a leaf list (dense per-level arrays)."""
 
     def __init__(self, nx0, ny0, nz0, max_level, tau0, bc, bc_u, x, y, z, level, state,
-                 rho_init=1.0, u_init=(0.0, 0.0, 0.0), rho0=1.0):
+                 rho_init=1.0, u_init=(0.0, 0.0, 0.0), rho0=1.0, single=False):
+        self.L = lib_f32() if single else lib()
         p = _LbmParams()
         p.nx0, p.ny0, p.nz0, p.max_level, p.tau0, p.rho0 = nx0, ny0, nz0, max_level, tau0, rho0
         for f in range(6):
@@ -372,7 +390,7 @@ class Lbm:
         self.state = np.ascontiguousarray(state, np.uint8)
         u = np.array(u_init, np.float64)
         err = C.c_int(0)
-        self.h = lib().or_lbm_create(C.byref(p), _p(self.x), _p(self.y), _p(self.z), _p(self.level),
+        self.h = self.L.or_lbm_create(C.byref(p), _p(self.x), _p(self.y), _p(self.z), _p(self.level),
                                      _p(self.state), len(self.x), float(rho_init), _p(u),
                                      C.byref(err))
         if not self.h:
@@ -381,43 +399,43 @@ class Lbm:
         self.fx, self.fy, self.fz, self.flv = self.x[fl], self.y[fl], self.z[fl], self.level[fl]
 
     def step(self, n=1):
-        lib().or_lbm_step(self.h, n)
+        self.L.or_lbm_step(self.h, n)
 
     def plant_coupling_error(self, c2f_mult, f2c_mult, slopes=1.0):
-        lib().or_lbm_plant_coupling_error(self.h, c2f_mult, f2c_mult, slopes)
+        self.L.or_lbm_plant_coupling_error(self.h, c2f_mult, f2c_mult, slopes)
 
     def get_f(self):
         out = np.empty((len(self.fx), 19), np.float64)
-        lib().or_lbm_get_f(self.h, _p(self.fx), _p(self.fy), _p(self.fz), _p(self.flv),
+        self.L.or_lbm_get_f(self.h, _p(self.fx), _p(self.fy), _p(self.fz), _p(self.flv),
                            len(self.fx), _p(out))
         return out
 
     def set_f(self, f):
         f = np.ascontiguousarray(f, np.float64)
-        lib().or_lbm_set_f(self.h, _p(self.fx), _p(self.fy), _p(self.fz), _p(self.flv),
+        self.L.or_lbm_set_f(self.h, _p(self.fx), _p(self.fy), _p(self.fz), _p(self.flv),
                            len(self.fx), _p(f))
 
     def moments(self):
         rho = np.empty(len(self.fx))
         u = np.empty((len(self.fx), 3))
-        lib().or_lbm_moments(self.h, _p(self.fx), _p(self.fy), _p(self.fz), _p(self.flv),
+        self.L.or_lbm_moments(self.h, _p(self.fx), _p(self.fy), _p(self.fz), _p(self.flv),
                              len(self.fx), _p(rho), _p(u))
         return rho, u
 
     def totals(self):
         m = np.zeros(1)
         j = np.zeros(3)
-        lib().or_lbm_totals(self.h, _p(m), _p(j))
+        self.L.or_lbm_totals(self.h, _p(m), _p(j))
         return float(m[0]), j
 
     def force(self):
         F = np.zeros(3)
-        lib().or_lbm_force(self.h, _p(F))
+        self.L.or_lbm_force(self.h, _p(F))
         return F
 
     def vorticity(self):
         out = np.empty(len(self.fx))
-        lib().or_lbm_vorticity(self.h, _p(self.fx), _p(self.fy), _p(self.fz), _p(self.flv), len(self.fx),
+        self.L.or_lbm_vorticity(self.h, _p(self.fx), _p(self.fy), _p(self.fz), _p(self.flv), len(self.fx),
                                _p(out))
         return out
 
@@ -427,10 +445,10 @@ class Lbm:
         level = np.ascontiguousarray(level, np.uint8)
         state = np.ascontiguousarray(state, np.uint8)
         err = C.c_int(0)
-        h = lib().or_lbm_regrid(self.h, _p(x), _p(y), _p(z), _p(level), _p(state), len(x), C.byref(err))
+        h = self.L.or_lbm_regrid(self.h, _p(x), _p(y), _p(z), _p(level), _p(state), len(x), C.byref(err))
         if not h:
             raise ValueError(f"oracle mesh error {err.value}")
-        lib().or_lbm_destroy(self.h)
+        self.L.or_lbm_destroy(self.h)
         self.h = h
         self.x, self.y, self.z, self.level, self.state = x, y, z, level, state
         fl = state == 0
@@ -439,7 +457,7 @@ class Lbm:
     def __del__(self):
         try:
             if self.h:
-                lib().or_lbm_destroy(self.h)
+                self.L.or_lbm_destroy(self.h)
                 self.h = None
         except Exception:
             pass
