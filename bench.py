@@ -9,12 +9,15 @@ on the diameter, a device regrid pass every 100 steps.  A "step" is one coarse
 (level-0) time step with all level sub-cycles; MLUPS counts every leaf
 update (level l leaves update 2^l times per step).
 
-Every regrid_every steps the timed loop runs a full device regrid pass over
-the leaves (refinement marks, then fl_regrid: D3Q19 balance, merge,
-compaction, split).  The sphere's mesh is static, so the pass returns the
-same leaves; the bench checks that it split and merged nothing, which is
-what lets it keep the solver's layout.  One rebuild of the solver's layout
-for the same leaves (fl_lbm_regrid) is timed beside the loop.
+Every regrid_every steps the timed loop runs a full regrid on the device:
+refinement marks, the regrid pass (fl_regrid: D3Q19 balance, merge,
+compaction, split), then the solver's lattice layout rebuilt by kernels
+from the pass's leaves and the populations carried over
+(fl_lbm_regrid_batch_device).  The sphere's mesh is static, so the pass
+returns the same leaves (the bench checks that it split and merged
+nothing) and the rebuild is done anyway, as a regrid that changed the mesh
+would do it; --regrid-keep-layout keeps the layout when the pass changed
+nothing.  The pass and the rebuild are also timed alone, before the loop.
 
 Prints one JSON line.  `value` is whole-job MLUPS; `roofline` is the level-3
 collide-stream kernel (the dominant kernel) measured with HIP events on the
@@ -113,6 +116,8 @@ def main():
     ap.add_argument("--scale", type=int, default=1, help="divide the base grid and radii by this")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--regrid-every", type=int, default=100)
+    ap.add_argument("--regrid-keep-layout", action="store_true",
+                    help="keep the solver's layout when the regrid pass changed no leaf")
     ap.add_argument("--no-graph", action="store_true", help="step eagerly instead of replaying a hipGraph")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default collide variant")
     ap.add_argument("--variant", default=None, help="force this collide variant on every level (no autotune)")
@@ -172,20 +177,32 @@ def main():
         same = rst.num_split == 0 and rst.num_groups == 0 and new.n == cells.n
         if not same:
             raise RuntimeError("the sphere's mesh changed at a regrid; a static mesh is expected")
+        return new
 
-    # untimed: one regrid pass alone, and one rebuild of the solver's layout
-    # for the same leaves (what a regrid that changes the mesh costs)
+    def full_regrid():
+        """The regrid pass, then the solver's layout rebuilt on the device
+        from its leaves with the populations carried over."""
+        new = regrid_pass()
+        if not args.regrid_keep_layout:
+            solver.regrid_device(new)
+            return 1
+        return 0
+
+    # untimed: one regrid pass alone, and one layout rebuild + population
+    # transfer from the pass's leaves alone
     _lib.check(L.fl_device_sync())
     t_r0 = time.perf_counter()
-    regrid_pass()
+    new = regrid_pass()
     _lib.check(L.fl_device_sync())
     regrid_pass_s = time.perf_counter() - t_r0
-    t_r0 = time.perf_counter()
-    solver.regrid(mesh["x"], mesh["y"], mesh["z"], mesh["level"], mesh["state"])
-    solver.sync()
-    layout_rebuild_s = time.perf_counter() - t_r0
-    log(f"regrid pass {regrid_pass_s * 1e3:.1f} ms; solver layout rebuild for the same leaves "
-        f"{layout_rebuild_s:.2f} s")
+    rebuild_ms = []
+    for _ in range(3):  # the first call also allocates what later ones reuse
+        t_r0 = time.perf_counter()
+        solver.regrid_device(new)
+        solver.sync()
+        rebuild_ms.append((time.perf_counter() - t_r0) * 1e3)
+    log(f"regrid pass {regrid_pass_s * 1e3:.1f} ms; solver layout rebuild + population transfer from "
+        f"the pass's leaves on the device: {', '.join(f'{t:.1f}' for t in rebuild_ms)} ms")
 
     solver.step(args.warmup)
     if not args.no_graph:
@@ -197,7 +214,7 @@ def main():
     _lib.check(L.fl_event_create(_lib.C.byref(ev0)))
     _lib.check(L.fl_event_create(_lib.C.byref(ev1)))
     _lib.check(L.fl_device_sync())
-    regrids = 0
+    regrids = rebuilds = 0
     w0 = time.perf_counter()
     _lib.check(L.fl_event_record(ev0, stream))
     k = 0
@@ -207,9 +224,9 @@ def main():
             n = min(n, args.regrid_every - k % args.regrid_every)
         solver.step(n)
         k += n
-        # the regrid pass after every regrid_every-th step, inside the timed region
+        # the full regrid after every regrid_every-th step, inside the timed region
         if args.regrid_every and k % args.regrid_every == 0:
-            regrid_pass()
+            rebuilds += full_regrid()
             regrids += 1
         if k % 50 == 0:
             log(f"step {k}/{args.steps}")
@@ -271,14 +288,18 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"flow past a sphere: base {nx0}x{ny0}x{nz0}, r={r:g} at {ctr}, "
                                f"{levels} levels within {2 * r:g} cells, Re={re:g}, u_in={u_in}, "
-                               f"tau0={tau0:.4f}, device regrid pass every {args.regrid_every} steps",
+                               f"tau0={tau0:.4f}, device regrid (pass"
+                               f"{'' if args.regrid_keep_layout else ' + layout rebuild + population transfer'}"
+                               f") every {args.regrid_every} steps",
                    "leaves_per_level": leaves, "updates_per_step": upd, "storage": args.storage,
                    "parallelism": "single GPU"},
         "hbm_gbps_algorithmic": round(mlups * 1e6 * bpu / 1e9, 1),
         "gpu_event_ms": round(ms.value, 3),
         "regrid_passes": regrids,
+        "layout_rebuilds": rebuilds,
         "regrid_pass_ms": round(regrid_pass_s * 1e3, 2),
-        "layout_rebuild_s": round(layout_rebuild_s, 3),
+        "layout_rebuild_ms": round(float(np.median(rebuild_ms)), 2),
+        "layout_rebuild_first_ms": round(rebuild_ms[0], 2),
         "collide_variants": [variant_names()[int(st2.variant[l])] for l in range(levels + 1)],
         "graph": bool(st2.graph),
         "drag_coefficient": round(float(cd), 4),

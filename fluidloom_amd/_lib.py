@@ -70,6 +70,13 @@ class LbmStats(C.Structure):
                 ("variant", i32 * 24), ("ninst", i32), ("graph", i32), ("graph_launches", u64)]
 
 
+class LbmLayout(C.Structure):
+    _fields_ = [(nm, C.c_void_p) for nm in ("nbr", "bcoord", "binst", "bflags", "cflags", "c2f_gchild", "c2f_gcap",
+                                            "c2f_gmask", "cap_slots", "f2c_ghost", "f2c_child", "force_links",
+                                            "abeg", "lbeg")] + \
+               [(nm, C.c_uint64) for nm in ("nb", "nactive", "n_groups", "n_cap", "n_f2c", "n_links")]
+
+
 _SIGS = {
     "fl_last_error_string": (C.c_char_p, []),
     "fl_version": (C.c_char_p, []),
@@ -87,6 +94,8 @@ _SIGS = {
     "fl_stream_destroy": (C.c_int, [vp]),
     "fl_stream_sync": (C.c_int, [vp]),
     "fl_device_sync": (C.c_int, []),
+    "fl_release_cached_memory": (C.c_int, []),
+    "fl_cached_memory_bytes": (C.c_int, [C.POINTER(C.c_uint64)]),
     "fl_event_create": (C.c_int, [C.POINTER(vp)]),
     "fl_event_destroy": (C.c_int, [vp]),
     "fl_event_record": (C.c_int, [vp, vp]),
@@ -153,6 +162,11 @@ _SIGS = {
     "fl_lbm_moments_inst": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, sz, vp, vp]),
     "fl_lbm_vorticity_inst": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, sz, vp]),
     "fl_lbm_regrid_batch": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+    "fl_lbm_regrid_batch_device": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+    "fl_lbm_create_batch_device": (C.c_int, [C.POINTER(LbmParams), C.c_int, vp, vp, vp, vp, vp, vp, vp,
+                                             C.POINTER(vp)]),
+    "fl_lbm_layout_get": (C.c_int, [vp, C.c_int, C.POINTER(LbmLayout)]),
+    "fl_lbm_layout_on_host": (C.c_int, [C.c_int]),
     "fl_lbm_totals_batch": (C.c_int, [vp, vp, vp]),
     "fl_lbm_force_batch": (C.c_int, [vp, vp]),
     "fl_lbm_velocity_change": (C.c_int, [vp, vp]),

@@ -15,6 +15,12 @@ void set_error(const std::string& msg);
 // Records a HIP failure; returns FL_E_HIP (or FL_E_OOM for out-of-memory).
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
 
+// Device blocks from the library's block cache (dev_cache.hip) on the
+// current device; cache_free keeps the block for later requests once stream
+// s (the stream of the block's last user) has drained.
+fl_status cache_alloc(void** p, size_t bytes);
+void cache_free(void* p, hipStream_t s);
+
 constexpr uint64_t kEmptyKey = 0xFFFFFFFFFFFFFFFFull;
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 constexpr uint32_t kFiner = 0xFFFFFFFEu;
@@ -36,6 +42,12 @@ inline unsigned grid_for(size_t n, unsigned block) {
     } while (0)
 
 #define FL_CHECK_LAUNCH() FL_HIP(hipGetLastError())
+
+#define FL_TRY(x)                \
+    do {                         \
+        fl_status fl_r_ = (x);   \
+        if (fl_r_) return fl_r_; \
+    } while (0)
 
 #define FL_REQUIRE(cond, msg)                                               \
     do {                                                                    \

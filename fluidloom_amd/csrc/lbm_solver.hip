@@ -35,6 +35,7 @@
 #include "fl_internal.hpp"
 #include "lbm_device.hpp"
 #include "lbm_kernels.hpp"
+#include "lbm_layout.hpp"
 
 namespace fl {
 namespace lbm {
@@ -162,6 +163,9 @@ struct HostLevel {
     std::vector<uint32_t> c2f_gchild, c2f_gcap, c2f_gmask;
     std::vector<uint32_t> abeg;  // [ninst+1] active-brick range of each instance
     std::vector<uint32_t> lbeg;  // [ninst+1] solid-link range of each instance
+    // list sizes (set by either builder; the host vectors above exist only
+    // when the layout was built on the host)
+    size_t n_c2f = 0, n_groups = 0, n_cap = 0, n_f2c = 0, n_links = 0;
 
     int32_t brick_at(int inst, int32_t bx, int32_t by, int32_t bz) const {
         const BrickGrid& g = grid[(size_t)inst];
@@ -198,6 +202,9 @@ struct DevLevel {
     uint32_t* force_links = nullptr;
     uint32_t* abeg = nullptr;
     uint32_t* lbeg = nullptr;
+    uint64_t* map_keys = nullptr;  // brick map: packed brick key -> brick (slot lookups)
+    uint32_t* map_vals = nullptr;
+    uint64_t map_cap = 0;
     float* itau = nullptr;
     double* itau_d = nullptr;
     double* bvals = nullptr;  // [max(nactive * 4, nlinks * 3)] reduction staging
@@ -221,6 +228,8 @@ size_t compute_bytes(int storage) { return storage == FL_STORE_F64 ? 8 : 4; }
 struct fl_lbm_solver {
     std::vector<fl::lbm::InstHost> inst;
     std::vector<fl::lbm::InstDev> hinst;
+    std::vector<fl::lbm::LayoutInst> linst;
+    fl::lbm::LayoutInst* dlinst = nullptr;
     int storage = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -256,7 +265,7 @@ struct fl_lbm_solver {
     }
     ~fl_lbm_solver() {
         drop_graphs();
-        for (void* a : allocs) (void)hipFree(a);
+        for (void* a : allocs) fl::cache_free(a, stream);
         if (qslots) (void)hipFree(qslots);
         if (qbuf) (void)hipFree(qbuf);
         if (own_ticks && ticks) (void)hipFree(ticks);
@@ -273,7 +282,7 @@ template <typename T>
 fl_status dev_upload(fl_lbm_solver* S, T** dst, const std::vector<T>& v) {
     *dst = nullptr;
     if (v.empty()) return FL_OK;
-    FL_HIP(hipMalloc(reinterpret_cast<void**>(dst), v.size() * sizeof(T)));
+    FL_TRY(fl::cache_alloc(reinterpret_cast<void**>(dst), v.size() * sizeof(T)));
     S->allocs.push_back(*dst);
     S->device_bytes += v.size() * sizeof(T);
     FL_HIP(hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, S->stream));
@@ -283,7 +292,7 @@ fl_status dev_upload(fl_lbm_solver* S, T** dst, const std::vector<T>& v) {
 fl_status dev_alloc(fl_lbm_solver* S, void** dst, size_t bytes) {
     *dst = nullptr;
     if (bytes == 0) return FL_OK;
-    FL_HIP(hipMalloc(dst, bytes));
+    FL_TRY(fl::cache_alloc(dst, bytes));
     S->allocs.push_back(*dst);
     S->device_bytes += bytes;
     return FL_OK;
@@ -749,6 +758,14 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
     for (int l = 0; l < NL; ++l)
         if ((uint64_t)S->h[l].nb * 64 >= (1ull << 32) / 32)
             return mesh_error("level too large for 32-bit slot lists");
+    for (int l = 0; l < NL; ++l) {
+        HostLevel& H = S->h[l];
+        H.n_c2f = H.c2f_ghost.size();
+        H.n_groups = H.c2f_gcap.size() / kC2FCaps;
+        H.n_cap = H.cap_slots.size();
+        H.n_f2c = H.f2c_ghost.size();
+        H.n_links = H.force_links.size();
+    }
     return FL_OK;
 }
 
@@ -766,17 +783,52 @@ uint64_t layout_bytes(const fl_lbm_solver* S) {
         t += (uint64_t)NI * (sizeof(float) + sizeof(double));
         t += std::max<uint64_t>((uint64_t)H.nactive * 4, (uint64_t)H.force_links.size() * 3) * 8;
         t += 2ull * H.nb * 64 * Q * sb + 2ull * cap_elems(H.cap_slots.size()) * cb;
+        t += brick_map_bytes(H.nb);
     }
     return t;
 }
 
+fl_status solver_alloc(fl_lbm_solver* S, void** p, size_t bytes) { return dev_alloc(S, p, bytes); }
+
+// Device tables of a layout built on the host, and the brick map that slot
+// lookups use with either builder.
+fl_status upload_host_tables(fl_lbm_solver* S, int l) {
+    HostLevel& H = S->h[l];
+    DevLevel& D = S->d[l];
+    fl_status r;
+    if ((r = dev_upload(S, &D.nbr, H.nbr))) return r;
+    if ((r = dev_upload(S, &D.bcoord, H.bcoord))) return r;
+    if ((r = dev_upload(S, &D.binst, H.binst))) return r;
+    if ((r = dev_upload(S, &D.bflags, H.bflags))) return r;
+    if ((r = dev_upload(S, &D.cflags, H.cflags))) return r;
+    if ((r = dev_upload(S, &D.c2f_gchild, H.c2f_gchild))) return r;
+    if ((r = dev_upload(S, &D.c2f_gcap, H.c2f_gcap))) return r;
+    if ((r = dev_upload(S, &D.c2f_gmask, H.c2f_gmask))) return r;
+    if ((r = dev_upload(S, &D.cap_slots, H.cap_slots))) return r;
+    if ((r = dev_upload(S, &D.f2c_ghost, H.f2c_ghost))) return r;
+    if ((r = dev_upload(S, &D.f2c_child, H.f2c_child))) return r;
+    if ((r = dev_upload(S, &D.force_links, H.force_links))) return r;
+    if ((r = dev_upload(S, &D.abeg, H.abeg))) return r;
+    if ((r = dev_upload(S, &D.lbeg, H.lbeg))) return r;
+    LayoutLevel M;
+    const DevAlloc alloc = [S](void** p, size_t b) { return solver_alloc(S, p, b); };
+    if ((r = build_brick_map(D.bcoord, D.binst, H.nb, S->stream, alloc, M))) return r;
+    D.map_keys = M.map_keys;
+    D.map_vals = M.map_vals;
+    D.map_cap = M.map_cap;
+    return FL_OK;
+}
+
+// Buffers, per-level constants and the initial state; the layout tables are
+// uploaded first unless the device builder made them.
 template <typename T>
-fl_status upload_and_init(fl_lbm_solver* S) {
+fl_status upload_and_init(fl_lbm_solver* S, bool tables_on_device) {
     const size_t sb = sizeof(T);
     const size_t cb = compute_bytes(S->storage);
     const int NI = S->ninst();
     fl_status r;
     if ((r = dev_upload(S, &S->dinst, S->hinst))) return r;
+    if ((r = dev_upload(S, &S->dlinst, S->linst))) return r;
     {
         void* a;
         if ((r = dev_alloc(S, &a, (size_t)NI * 4 * sizeof(double)))) return r;
@@ -785,20 +837,7 @@ fl_status upload_and_init(fl_lbm_solver* S) {
     for (int l = 0; l < S->L; ++l) {
         HostLevel& H = S->h[l];
         DevLevel& D = S->d[l];
-        if ((r = dev_upload(S, &D.nbr, H.nbr))) return r;
-        if ((r = dev_upload(S, &D.bcoord, H.bcoord))) return r;
-        if ((r = dev_upload(S, &D.binst, H.binst))) return r;
-        if ((r = dev_upload(S, &D.bflags, H.bflags))) return r;
-        if ((r = dev_upload(S, &D.cflags, H.cflags))) return r;
-        if ((r = dev_upload(S, &D.c2f_gchild, H.c2f_gchild))) return r;
-        if ((r = dev_upload(S, &D.c2f_gcap, H.c2f_gcap))) return r;
-        if ((r = dev_upload(S, &D.c2f_gmask, H.c2f_gmask))) return r;
-        if ((r = dev_upload(S, &D.cap_slots, H.cap_slots))) return r;
-        if ((r = dev_upload(S, &D.f2c_ghost, H.f2c_ghost))) return r;
-        if ((r = dev_upload(S, &D.f2c_child, H.f2c_child))) return r;
-        if ((r = dev_upload(S, &D.force_links, H.force_links))) return r;
-        if ((r = dev_upload(S, &D.abeg, H.abeg))) return r;
-        if ((r = dev_upload(S, &D.lbeg, H.lbeg))) return r;
+        if (!tables_on_device && (r = upload_host_tables(S, l))) return r;
         std::vector<float> itf(NI);
         std::vector<double> itd(NI);
         for (int k = 0; k < NI; ++k) {
@@ -808,13 +847,13 @@ fl_status upload_and_init(fl_lbm_solver* S) {
         if ((r = dev_upload(S, &D.itau, itf))) return r;
         if ((r = dev_upload(S, &D.itau_d, itd))) return r;
         void* bv;
-        const size_t nbv = std::max<size_t>((size_t)H.nactive * 4, H.force_links.size() * 3);
+        const size_t nbv = std::max<size_t>((size_t)H.nactive * 4, H.n_links * 3);
         if ((r = dev_alloc(S, &bv, nbv * sizeof(double)))) return r;
         D.bvals = static_cast<double*>(bv);
         const size_t slots = (size_t)H.nb * 64;
         for (int k = 0; k < 2; ++k) {
             if ((r = dev_alloc(S, &D.buf[k], slots * Q * sb))) return r;
-            if ((r = dev_alloc(S, &D.cap[k], cap_elems(H.cap_slots.size()) * cb))) return r;
+            if ((r = dev_alloc(S, &D.cap[k], cap_elems(H.n_cap) * cb))) return r;
         }
         D.cur = 0;
         D.ld.nbr = D.nbr;
@@ -844,10 +883,10 @@ fl_status advance(fl_lbm_solver* S, int l) {
     bool fine = false;
     for (int k = l + 1; k < S->L; ++k) fine = fine || S->h[k].nfluid > 0;
     hipStream_t st = S->stream;
-    const uint32_t ncap = (uint32_t)H.cap_slots.size();
+    const uint32_t ncap = (uint32_t)H.n_cap;
     if (fine) {
         DevLevel& F = S->d[l + 1];
-        FL_HIP(launch_fill_f2c<T>(D.ld, D.f2c_ghost, D.f2c_child, (uint32_t)H.f2c_ghost.size(), F.ld,
+        FL_HIP(launch_fill_f2c<T>(D.ld, D.f2c_ghost, D.f2c_child, (uint32_t)H.n_f2c, F.ld,
                                   static_cast<const T*>(F.buf[1 - F.cur]), static_cast<T*>(D.buf[D.cur]), st));
         FL_HIP(launch_capture<T>(D.ld, static_cast<const T*>(D.buf[1 - D.cur]), D.cap_slots, ncap, D.cap[0], st));
     }
@@ -860,7 +899,7 @@ fl_status advance(fl_lbm_solver* S, int l) {
         DevLevel& F = S->d[l + 1];
         const HostLevel& HF = S->h[l + 1];
         for (int sub = 0; sub < 2; ++sub) {
-            FL_HIP(launch_fill_c2f<T>(F.ld, F.c2f_gchild, F.c2f_gcap, F.c2f_gmask, (uint32_t)(HF.c2f_gcap.size() / kC2FCaps), D.cap[0], D.cap[1],
+            FL_HIP(launch_fill_c2f<T>(F.ld, F.c2f_gchild, F.c2f_gcap, F.c2f_gmask, (uint32_t)HF.n_groups, D.cap[0], D.cap[1],
                                       ncap, 0.5 * sub, static_cast<T*>(F.buf[F.cur]), st));
             fl_status r = advance<T>(S, l + 1);
             if (r) return r;
@@ -923,15 +962,42 @@ fl_status dispatch_step(fl_lbm_solver* S, int nsteps) {
     return FL_OK;
 }
 
-// host slots of the query leaves of one instance; -1 when not a fluid leaf
-void query_slots(fl_lbm_solver* S, int inst, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv,
-                 size_t n, std::vector<int64_t>& slots) {
-    slots.resize(n);
-    for (size_t k = 0; k < n; ++k) {
-        int l = lv[k];
-        slots[k] = l < S->L ? S->h[l].slot_at(inst, x[k], y[k], z[k]) : -1;
-        if (slots[k] >= 0 && S->h[l].stype[(size_t)slots[k]] != T_FLUID) slots[k] = -1;
+LevelMaps level_maps(const fl_lbm_solver* S) {
+    LevelMaps m{};
+    m.nlevels = S->L;
+    for (int l = 0; l < S->L; ++l) {
+        const DevLevel& D = S->d[l];
+        m.m[l] = LevelMap{D.map_keys, D.map_vals, D.map_cap, D.cflags};
     }
+    return m;
+}
+
+// slots of the query leaves of one instance (looked up on the device in the
+// brick maps); -1 when not a fluid leaf
+fl_status query_slots(fl_lbm_solver* S, int inst, const int32_t* x, const int32_t* y, const int32_t* z,
+                      const uint8_t* lv, size_t n, std::vector<int64_t>& slots) {
+    slots.assign(n, -1);
+    if (n == 0) return FL_OK;
+    struct Buf {
+        void* p = nullptr;
+        ~Buf() { if (p) (void)hipFree(p); }
+    } bx, by, bz, bl, bo;
+    FL_HIP(hipMalloc(&bx.p, n * 4));
+    FL_HIP(hipMalloc(&by.p, n * 4));
+    FL_HIP(hipMalloc(&bz.p, n * 4));
+    FL_HIP(hipMalloc(&bl.p, n));
+    FL_HIP(hipMalloc(&bo.p, n * 8));
+    FL_HIP(hipMemcpyAsync(bx.p, x, n * 4, hipMemcpyHostToDevice, S->stream));
+    FL_HIP(hipMemcpyAsync(by.p, y, n * 4, hipMemcpyHostToDevice, S->stream));
+    FL_HIP(hipMemcpyAsync(bz.p, z, n * 4, hipMemcpyHostToDevice, S->stream));
+    FL_HIP(hipMemcpyAsync(bl.p, lv, n, hipMemcpyHostToDevice, S->stream));
+    fl_status r = launch_query_slots(level_maps(S), S->dlinst, inst, static_cast<const int32_t*>(bx.p),
+                                     static_cast<const int32_t*>(by.p), static_cast<const int32_t*>(bz.p),
+                                     static_cast<const uint8_t*>(bl.p), n, static_cast<int64_t*>(bo.p), S->stream);
+    if (r) return r;
+    FL_HIP(hipMemcpyAsync(slots.data(), bo.p, n * 8, hipMemcpyDeviceToHost, S->stream));
+    FL_HIP(hipStreamSynchronize(S->stream));
+    return FL_OK;
 }
 
 fl_status ensure_query(fl_lbm_solver* S, size_t n) {
@@ -951,7 +1017,10 @@ template <typename T>
 fl_status get_f(fl_lbm_solver* S, int inst, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv,
                 size_t n, double* out) {
     std::vector<int64_t> slots;
-    query_slots(S, inst, x, y, z, lv, n, slots);
+    {
+        fl_status qr = query_slots(S, inst, x, y, z, lv, n, slots);
+        if (qr) return qr;
+    }
     std::vector<double> tmp;
     for (size_t k = 0; k < n; ++k)
         if (lv[k] >= S->L)
@@ -982,7 +1051,10 @@ template <typename T>
 fl_status set_f(fl_lbm_solver* S, int inst, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv,
                 size_t n, const double* in) {
     std::vector<int64_t> slots;
-    query_slots(S, inst, x, y, z, lv, n, slots);
+    {
+        fl_status qr = query_slots(S, inst, x, y, z, lv, n, slots);
+        if (qr) return qr;
+    }
     for (int l = 0; l < S->L; ++l) {
         std::vector<int64_t> ls;
         std::vector<double> vals;
@@ -1033,7 +1105,7 @@ fl_status force(fl_lbm_solver* S, double* out3) {
     bool first = true;
     for (int l = 0; l < S->L; ++l) {
         DevLevel& D = S->d[l];
-        const uint32_t nl = (uint32_t)S->h[l].force_links.size();
+        const uint32_t nl = (uint32_t)S->h[l].n_links;
         if (nl == 0) continue;
         FL_HIP(launch_link_force<T>(D.force_links, nl, D.ld.S, static_cast<const T*>(D.buf[D.cur]),
                                     std::ldexp(1.0, -2 * l), D.bvals, S->stream));
@@ -1146,15 +1218,12 @@ InstDev make_inst_dev(const fl_lbm_params& p) {
     return I;
 }
 
-// Host half of creation: validation and the layout (no device work).
-fl_status plan_solver(const fl_lbm_params* params, int ninst, const size_t* offsets, const int32_t* h_x,
-                      const int32_t* h_y, const int32_t* h_z, const uint8_t* h_level, const uint8_t* h_state,
-                      std::unique_ptr<fl_lbm_solver>& out) {
+// Instances of a batch: parameter checks and the per-instance records.
+fl_status init_instances(const fl_lbm_params* params, int ninst, const size_t* offsets,
+                         std::unique_ptr<fl_lbm_solver>& out) {
     FL_REQUIRE(params && offsets, "null pointer");
     FL_REQUIRE(ninst >= 1 && ninst <= 65535, "instance count must be in 1..65535");
-    const size_t n = offsets[ninst];
     FL_REQUIRE(offsets[0] == 0, "offsets[0] must be 0");
-    FL_REQUIRE(n == 0 || (h_x && h_y && h_z && h_level && h_state), "null leaf arrays");
     std::unique_ptr<fl_lbm_solver> S(new fl_lbm_solver());
     S->storage = params[0].storage;
     int maxl = 0;
@@ -1169,21 +1238,90 @@ fl_status plan_solver(const fl_lbm_params* params, int ninst, const size_t* offs
         ih.count = offsets[k + 1] - offsets[k];
         S->inst.push_back(ih);
         S->hinst.push_back(make_inst_dev(params[k]));
+        LayoutInst li{};
+        li.ext0[0] = params[k].nx0;
+        li.ext0[1] = params[k].ny0;
+        li.ext0[2] = params[k].nz0;
+        li.max_level = params[k].max_level;
+        for (int f = 0; f < 6; ++f) li.bc[f] = params[k].bc[f];
+        S->linst.push_back(li);
         maxl = std::max(maxl, params[k].max_level);
     }
     S->L = maxl + 1;
-    fl_status r = build_layout(S.get(), h_x, h_y, h_z, h_level, h_state, n);
+    out = std::move(S);
+    return FL_OK;
+}
+
+// Host half of creation with the host builder: validation and the layout
+// (no device work).  fl_lbm_plan_bytes uses it; so does creation when the
+// host builder is selected (fl_lbm_layout_on_host).
+fl_status plan_solver(const fl_lbm_params* params, int ninst, const size_t* offsets, const int32_t* h_x,
+                      const int32_t* h_y, const int32_t* h_z, const uint8_t* h_level, const uint8_t* h_state,
+                      std::unique_ptr<fl_lbm_solver>& out) {
+    std::unique_ptr<fl_lbm_solver> S;
+    fl_status r = init_instances(params, ninst, offsets, S);
+    if (r) return r;
+    const size_t n = offsets[ninst];
+    FL_REQUIRE(n == 0 || (h_x && h_y && h_z && h_level && h_state), "null leaf arrays");
+    r = build_layout(S.get(), h_x, h_y, h_z, h_level, h_state, n);
     if (r) return r;
     out = std::move(S);
     return FL_OK;
 }
 
-fl_status create_solver(const fl_lbm_params* params, int ninst, const size_t* offsets, const int32_t* h_x,
-                        const int32_t* h_y, const int32_t* h_z, const uint8_t* h_level, const uint8_t* h_state,
-                        hipStream_t stream, std::unique_ptr<fl_lbm_solver>& out) {
-    std::unique_ptr<fl_lbm_solver> S;
-    fl_status r = plan_solver(params, ninst, offsets, h_x, h_y, h_z, h_level, h_state, S);
+// The device builder: every table made from device leaf arrays; only sizes
+// and per-instance counts come back to the host.
+fl_status device_layout(fl_lbm_solver* S, const int32_t* d_x, const int32_t* d_y, const int32_t* d_z,
+                        const uint8_t* d_level, const uint8_t* d_state, size_t n) {
+    std::vector<size_t> offs(S->ninst() + 1, 0);
+    for (int k = 0; k < S->ninst(); ++k) offs[k + 1] = S->inst[k].first + S->inst[k].count;
+    std::vector<LayoutLevel> lv(S->L);
+    std::string err;
+    const DevAlloc alloc = [S](void** p, size_t b) { return solver_alloc(S, p, b); };
+    fl_status r = build_layout_device(S->linst, S->L, offs.data(), d_x, d_y, d_z, d_level, d_state, n, S->stream,
+                                      alloc, lv.data(), &err);
+    if (r == FL_E_MESH) return mesh_error(err);
     if (r) return r;
+    for (int l = 0; l < S->L; ++l) {
+        const LayoutLevel& Lv = lv[l];
+        HostLevel& H = S->h[l];
+        DevLevel& D = S->d[l];
+        H.level = l;
+        H.nb = Lv.nb;
+        H.nactive = Lv.nactive;
+        H.nfluid = Lv.nfluid;
+        H.nfluid_inst = Lv.nfluid_inst;
+        H.n_c2f = Lv.n_c2f;
+        H.n_groups = Lv.n_groups;
+        H.n_cap = Lv.n_cap;
+        H.n_f2c = Lv.n_f2c;
+        H.n_links = Lv.n_links;
+        H.abeg = Lv.abeg_h;
+        H.lbeg = Lv.lbeg_h;
+        D.nbr = Lv.nbr;
+        D.bcoord = Lv.bcoord;
+        D.binst = Lv.binst;
+        D.bflags = Lv.bflags;
+        D.cflags = Lv.cflags;
+        D.c2f_gchild = Lv.c2f_gchild;
+        D.c2f_gcap = Lv.c2f_gcap;
+        D.c2f_gmask = Lv.c2f_gmask;
+        D.cap_slots = Lv.cap_slots;
+        D.f2c_ghost = Lv.f2c_ghost;
+        D.f2c_child = Lv.f2c_child;
+        D.force_links = Lv.force_links;
+        D.abeg = Lv.abeg;
+        D.lbeg = Lv.lbeg;
+        D.map_keys = Lv.map_keys;
+        D.map_vals = Lv.map_vals;
+        D.map_cap = Lv.map_cap;
+    }
+    return FL_OK;
+}
+
+bool g_host_layout = false;  // fl_lbm_layout_on_host
+
+fl_status attach_stream(fl_lbm_solver* S, hipStream_t stream) {
     if (stream) {
         S->stream = stream;
     } else {
@@ -1192,25 +1330,108 @@ fl_status create_solver(const fl_lbm_params* params, int ninst, const size_t* of
         FL_HIP(hipStreamCreate(&S->stream));
         S->own_stream = true;
     }
+    return FL_OK;
+}
+
+// Stream and rounding clock of a new solver: its own, or for a regrid the
+// previous layout's (with its instances' live flags).
+fl_status attach(fl_lbm_solver* S, hipStream_t stream, const fl_lbm_solver* prev) {
+    if (prev) {
+        S->stream = prev->stream;
+        S->ticks = prev->ticks;
+        S->hinst = prev->hinst;
+        return FL_OK;
+    }
+    fl_status r = attach_stream(S, stream);
+    if (r) return r;
     FL_HIP(hipMalloc(reinterpret_cast<void**>(&S->ticks), 24 * sizeof(uint32_t)));
     S->own_ticks = true;
     FL_HIP(hipMemsetAsync(S->ticks, 0, 24 * sizeof(uint32_t), S->stream));
+    return FL_OK;
+}
+
+fl_status init_any(fl_lbm_solver* S, bool tables_on_device) {
     switch (S->storage) {
-        case FL_STORE_F32: r = upload_and_init<float>(S.get()); break;
-        case FL_STORE_F16: r = upload_and_init<_Float16>(S.get()); break;
-        case FL_STORE_F16C: r = upload_and_init<fp16c>(S.get()); break;
-        default: r = upload_and_init<double>(S.get()); break;
+        case FL_STORE_F32: return upload_and_init<float>(S, tables_on_device);
+        case FL_STORE_F16: return upload_and_init<_Float16>(S, tables_on_device);
+        case FL_STORE_F16C: return upload_and_init<fp16c>(S, tables_on_device);
+        default: return upload_and_init<double>(S, tables_on_device);
     }
+}
+
+// A solver (layout, buffers, initial state) for leaves on the device.
+fl_status build_on_device(const fl_lbm_params* params, int ninst, const size_t* offsets, const int32_t* d_x,
+                          const int32_t* d_y, const int32_t* d_z, const uint8_t* d_level, const uint8_t* d_state,
+                          hipStream_t stream, const fl_lbm_solver* prev, std::unique_ptr<fl_lbm_solver>& out) {
+    std::unique_ptr<fl_lbm_solver> S;
+    fl_status r = init_instances(params, ninst, offsets, S);
     if (r) return r;
+    const size_t n = offsets[ninst];
+    FL_REQUIRE(n == 0 || (d_x && d_y && d_z && d_level && d_state), "null leaf arrays");
+    if ((r = attach(S.get(), stream, prev))) return r;
+    if ((r = device_layout(S.get(), d_x, d_y, d_z, d_level, d_state, n))) return r;
+    if ((r = init_any(S.get(), true))) return r;
     out = std::move(S);
     return FL_OK;
+}
+
+// The same with the host builder (the reference the device layout is tested
+// against; leaves on the host).
+fl_status build_on_host(const fl_lbm_params* params, int ninst, const size_t* offsets, const int32_t* h_x,
+                        const int32_t* h_y, const int32_t* h_z, const uint8_t* h_level, const uint8_t* h_state,
+                        hipStream_t stream, const fl_lbm_solver* prev, std::unique_ptr<fl_lbm_solver>& out) {
+    std::unique_ptr<fl_lbm_solver> S;
+    fl_status r = plan_solver(params, ninst, offsets, h_x, h_y, h_z, h_level, h_state, S);
+    if (r) return r;
+    if ((r = attach(S.get(), stream, prev))) return r;
+    if ((r = init_any(S.get(), false))) return r;
+    out = std::move(S);
+    return FL_OK;
+}
+
+// Leaves given on the host: the host builder when selected, otherwise one
+// upload and the device builder.
+fl_status build_from_host(const fl_lbm_params* params, int ninst, const size_t* offsets, const int32_t* h_x,
+                          const int32_t* h_y, const int32_t* h_z, const uint8_t* h_level, const uint8_t* h_state,
+                          hipStream_t stream, const fl_lbm_solver* prev, std::unique_ptr<fl_lbm_solver>& out) {
+    FL_REQUIRE(params && offsets && ninst >= 1, "null pointer");
+    if (g_host_layout)
+        return build_on_host(params, ninst, offsets, h_x, h_y, h_z, h_level, h_state, stream, prev, out);
+    const size_t n = offsets[ninst];
+    FL_REQUIRE(n == 0 || (h_x && h_y && h_z && h_level && h_state), "null leaf arrays");
+    struct Buf {
+        void* p = nullptr;
+        ~Buf() { if (p) (void)hipFree(p); }
+    } bx, by, bz, bl, bs;
+    const size_t m = n ? n : 1;
+    FL_HIP(hipMalloc(&bx.p, m * 4));
+    FL_HIP(hipMalloc(&by.p, m * 4));
+    FL_HIP(hipMalloc(&bz.p, m * 4));
+    FL_HIP(hipMalloc(&bl.p, m));
+    FL_HIP(hipMalloc(&bs.p, m));
+    if (n) {
+        FL_HIP(hipMemcpy(bx.p, h_x, n * 4, hipMemcpyHostToDevice));
+        FL_HIP(hipMemcpy(by.p, h_y, n * 4, hipMemcpyHostToDevice));
+        FL_HIP(hipMemcpy(bz.p, h_z, n * 4, hipMemcpyHostToDevice));
+        FL_HIP(hipMemcpy(bl.p, h_level, n, hipMemcpyHostToDevice));
+        FL_HIP(hipMemcpy(bs.p, h_state, n, hipMemcpyHostToDevice));
+    }
+    fl_status r = build_on_device(params, ninst, offsets, static_cast<const int32_t*>(bx.p),
+                                  static_cast<const int32_t*>(by.p), static_cast<const int32_t*>(bz.p),
+                                  static_cast<const uint8_t*>(bl.p), static_cast<const uint8_t*>(bs.p), stream, prev,
+                                  out);
+    if (out && out->stream) (void)hipStreamSynchronize(out->stream);
+    return r;
 }
 
 template <typename T>
 fl_status vorticity(fl_lbm_solver* S, int inst, const int32_t* x, const int32_t* y, const int32_t* z,
                     const uint8_t* lv, size_t n, double* out) {
     std::vector<int64_t> slots;
-    query_slots(S, inst, x, y, z, lv, n, slots);
+    {
+        fl_status qr = query_slots(S, inst, x, y, z, lv, n, slots);
+        if (qr) return qr;
+    }
     for (size_t k = 0; k < n; ++k) out[k] = 0.0;
     std::vector<double> tmp;
     for (int l = 0; l < S->L; ++l) {
@@ -1234,74 +1455,32 @@ fl_status vorticity(fl_lbm_solver* S, int inst, const int32_t* x, const int32_t*
 }
 
 // Population transfer from the old solver O into the freshly built N (same
-// instances, new leaves).
+// instances, new leaves).  The transfer lists come from the device
+// (build_remap_lists: lookups in O's brick maps).
 template <typename T>
 fl_status remap(fl_lbm_solver* O, fl_lbm_solver* N) {
     std::vector<void*> tmp_allocs;
     fl_status st = FL_OK;
-    auto up = [&](const std::vector<uint32_t>& v) -> const uint32_t* {
-        if (v.empty() || st) return nullptr;
-        void* d = nullptr;
-        hipError_t e = hipMalloc(&d, v.size() * 4);
-        if (e != hipSuccess) {
-            st = fl::hip_fail(e, "remap lists", __FILE__, __LINE__);
-            return nullptr;
-        }
-        tmp_allocs.push_back(d);
-        e = hipMemcpyAsync(d, v.data(), v.size() * 4, hipMemcpyHostToDevice, N->stream);
-        if (e != hipSuccess) st = fl::hip_fail(e, "remap lists", __FILE__, __LINE__);
-        return static_cast<const uint32_t*>(d);
+    const DevAlloc talloc = [&tmp_allocs](void** p, size_t b) -> fl_status {
+        *p = nullptr;
+        if (b == 0) return FL_OK;
+        FL_TRY(fl::cache_alloc(p, b));
+        tmp_allocs.push_back(*p);
+        return FL_OK;
     };
-    // host lists must outlive the asynchronous copies
-    std::vector<std::vector<uint32_t>> keep;
+    const LevelMaps om = level_maps(O);
     for (int l = 0; l < N->L && st == FL_OK; ++l) {
         const HostLevel& H = N->h[l];
-        std::vector<uint32_t> cdst, csrc, sdst, ssrc, mdst, msrc, rdst;
-        for (uint32_t b = 0; b < H.nactive; ++b) {
-            const int in = H.binst[b];
-            for (int loc = 0; loc < 64; ++loc) {
-                const size_t s = (size_t)b * 64 + loc;
-                if (H.stype[s] != T_FLUID) continue;
-                const int32_t x = H.bcoord[3 * b] * 4 + (loc & 3), y = H.bcoord[3 * b + 1] * 4 + ((loc >> 2) & 3),
-                              z = H.bcoord[3 * b + 2] * 4 + (loc >> 4);
-                if (l < O->L) {
-                    const HostLevel& OH = O->h[l];
-                    int64_t os = OH.slot_at(in, x, y, z);
-                    if (os >= 0 && OH.stype[(size_t)os] == T_FLUID) {
-                        cdst.push_back((uint32_t)s);
-                        csrc.push_back((uint32_t)os);
-                        continue;
-                    }
-                }
-                if (l > 0 && l - 1 < O->L) {
-                    const HostLevel& OC = O->h[l - 1];
-                    int64_t ps = OC.slot_at(in, x >> 1, y >> 1, z >> 1);
-                    if (ps >= 0 && OC.stype[(size_t)ps] == T_FLUID) {
-                        sdst.push_back((uint32_t)s);
-                        ssrc.push_back((uint32_t)ps);
-                        continue;
-                    }
-                }
-                if (l + 1 < O->L) {
-                    const HostLevel& OF = O->h[l + 1];
-                    uint32_t ch[8];
-                    bool all = true;
-                    for (int q = 0; q < 8 && all; ++q) {
-                        int64_t c = OF.slot_at(in, 2 * x + (q & 1), 2 * y + ((q >> 1) & 1), 2 * z + ((q >> 2) & 1));
-                        all = c >= 0 && OF.stype[(size_t)c] == T_FLUID;
-                        if (all) ch[q] = (uint32_t)c;
-                    }
-                    if (all) {
-                        mdst.push_back((uint32_t)s);
-                        msrc.insert(msrc.end(), ch, ch + 8);
-                        continue;
-                    }
-                }
-                rdst.push_back((uint32_t)s);
-            }
-        }
-        RemapArgs<T> a;
         DevLevel& D = N->d[l];
+        LayoutLevel nl;
+        nl.nactive = H.nactive;
+        nl.bcoord = D.bcoord;
+        nl.binst = D.binst;
+        nl.cflags = D.cflags;
+        RemapLists R;
+        st = build_remap_lists(om, N->dlinst, l, nl, N->stream, talloc, R);
+        if (st) break;
+        RemapArgs<T> a;
         a.Ln = D.ld;
         a.A = static_cast<T*>(D.buf[0]);
         a.B = static_cast<T*>(D.buf[1]);
@@ -1310,38 +1489,30 @@ fl_status remap(fl_lbm_solver* O, fl_lbm_solver* N) {
             a.copy_src_buf = static_cast<const T*>(OD.buf[OD.cur]);
             a.So = OD.ld.S;
         }
-        a.copy_dst = up(cdst);
-        a.copy_src = up(csrc);
-        a.ncopy = (uint32_t)cdst.size();
+        a.copy_dst = R.copy_dst;
+        a.copy_src = R.copy_src;
+        a.ncopy = R.ncopy;
         if (l > 0 && l - 1 < O->L) {
             const DevLevel& OC = O->d[l - 1];
             a.Lc = OC.ld;
             a.split_prev = static_cast<const T*>(OC.buf[1 - OC.cur]);
         }
-        a.split_dst = up(sdst);
-        a.split_src = up(ssrc);
-        a.nsplit = (uint32_t)sdst.size();
+        a.split_dst = R.split_dst;
+        a.split_src = R.split_src;
+        a.nsplit = R.nsplit;
         if (l + 1 < O->L) {
             const DevLevel& OF = O->d[l + 1];
             a.Lf = OF.ld;
             a.merge_prev = static_cast<const T*>(OF.buf[1 - OF.cur]);
         }
-        a.merge_dst = up(mdst);
-        a.merge_src8 = up(msrc);
-        a.nmerge = (uint32_t)mdst.size();
-        a.rest_dst = up(rdst);
-        a.nrest = (uint32_t)rdst.size();
-        if (st) break;
+        a.merge_dst = R.merge_dst;
+        a.merge_src8 = R.merge_src8;
+        a.nmerge = R.nmerge;
+        a.rest_dst = R.rest_dst;
+        a.nrest = R.nrest;
         hipError_t e = launch_remap<T>(a, N->stream);
         if (e != hipSuccess) st = fl::hip_fail(e, "launch_remap", __FILE__, __LINE__);
         D.cur = 0;
-        keep.push_back(std::move(cdst));
-        keep.push_back(std::move(csrc));
-        keep.push_back(std::move(sdst));
-        keep.push_back(std::move(ssrc));
-        keep.push_back(std::move(mdst));
-        keep.push_back(std::move(msrc));
-        keep.push_back(std::move(rdst));
     }
     for (int l = 0; l < N->L && st == FL_OK; ++l) {
         const HostLevel& H = N->h[l];
@@ -1351,13 +1522,13 @@ fl_status remap(fl_lbm_solver* O, fl_lbm_solver* N) {
         const uint32_t* capsl = l > 0 ? N->d[l - 1].cap_slots : nullptr;
         const T* Af = l + 1 < N->L ? static_cast<const T*>(N->d[l + 1].buf[0]) : nullptr;
         const size_t Sf = l + 1 < N->L ? N->d[l + 1].ld.S : 0;
-        hipError_t e = launch_ghost_eq<T>(D.c2f_gchild, D.c2f_gcap, capsl, (uint32_t)(H.c2f_gcap.size() / kC2FCaps), Ac, Sc,
-                                          D.f2c_ghost, D.f2c_child, (uint32_t)H.f2c_ghost.size(), Af, Sf,
+        hipError_t e = launch_ghost_eq<T>(D.c2f_gchild, D.c2f_gcap, capsl, (uint32_t)H.n_groups, Ac, Sc,
+                                          D.f2c_ghost, D.f2c_child, (uint32_t)H.n_f2c, Af, Sf,
                                           static_cast<T*>(D.buf[0]), static_cast<T*>(D.buf[1]), D.ld.S, N->stream);
         if (e != hipSuccess) st = fl::hip_fail(e, "launch_ghost_eq", __FILE__, __LINE__);
     }
     hipError_t e = hipStreamSynchronize(N->stream);
-    for (void* p : tmp_allocs) (void)hipFree(p);
+    for (void* p : tmp_allocs) fl::cache_free(p, N->stream);
     if (st) return st;
     if (e != hipSuccess) return fl::hip_fail(e, "remap", __FILE__, __LINE__);
     return FL_OK;
@@ -1415,6 +1586,39 @@ using namespace fl::lbm;
     FL_REQUIRE(S, "null solver"); \
     FL_REQUIRE((inst) >= 0 && (inst) < (S)->ninst(), "instance index out of range")
 
+namespace fl {
+namespace lbm {
+// Populations of S carried over to the new layout N, which then replaces S's.
+fl_status adopt_layout(fl_lbm_solver* S, std::unique_ptr<fl_lbm_solver>& N) {
+    fl_status r;
+    switch (S->storage) {
+        case FL_STORE_F32: r = remap<float>(S, N.get()); break;
+        case FL_STORE_F16: r = remap<_Float16>(S, N.get()); break;
+        case FL_STORE_F16C: r = remap<fp16c>(S, N.get()); break;
+        default: r = remap<double>(S, N.get()); break;
+    }
+    if (r) return r;
+    // the handle keeps its identity, stream, variants and graph setting; its
+    // layout becomes the new mesh (the old one is freed with N)
+    S->drop_graphs();
+    std::swap(S->inst, N->inst);
+    std::swap(S->L, N->L);
+    for (int l = 0; l < 24; ++l) {
+        std::swap(S->h[l], N->h[l]);
+        std::swap(S->d[l], N->d[l]);
+    }
+    std::swap(S->dinst, N->dinst);
+    std::swap(S->dlinst, N->dlinst);
+    std::swap(S->acc, N->acc);
+    std::swap(S->allocs, N->allocs);
+    std::swap(S->device_bytes, N->device_bytes);
+    // the velocity snapshots and window sums belonged to the old layout
+    S->nsamples = 0;
+    return FL_OK;
+}
+}  // namespace lbm
+}  // namespace fl
+
 extern "C" {
 
 fl_status fl_lbm_create(const fl_lbm_params* p, const int32_t* h_x, const int32_t* h_y, const int32_t* h_z,
@@ -1431,9 +1635,55 @@ fl_status fl_lbm_create_batch(const fl_lbm_params* params, int ninst, const size
     FL_REQUIRE(params && offsets && out, "null pointer");
     *out = nullptr;
     std::unique_ptr<fl_lbm_solver> S;
-    fl_status r = create_solver(params, ninst, offsets, h_x, h_y, h_z, h_level, h_state, FL_STREAM(s), S);
+    fl_status r = build_from_host(params, ninst, offsets, h_x, h_y, h_z, h_level, h_state, FL_STREAM(s), nullptr, S);
     if (r) return r;
     *out = S.release();
+    return FL_OK;
+}
+
+fl_status fl_lbm_create_batch_device(const fl_lbm_params* params, int ninst, const size_t* offsets,
+                                     const int32_t* d_x, const int32_t* d_y, const int32_t* d_z,
+                                     const uint8_t* d_level, const uint8_t* d_state, fl_stream s,
+                                     fl_lbm_solver** out) {
+    FL_REQUIRE(params && offsets && out, "null pointer");
+    *out = nullptr;
+    std::unique_ptr<fl_lbm_solver> S;
+    fl_status r = build_on_device(params, ninst, offsets, d_x, d_y, d_z, d_level, d_state, FL_STREAM(s), nullptr, S);
+    if (r) return r;
+    *out = S.release();
+    return FL_OK;
+}
+
+fl_status fl_lbm_layout_on_host(int on) {
+    g_host_layout = on != 0;
+    return FL_OK;
+}
+
+fl_status fl_lbm_layout_get(const fl_lbm_solver* S, int level, fl_lbm_layout* out) {
+    FL_REQUIRE(S && out, "null pointer");
+    FL_REQUIRE(level >= 0 && level < S->L, "level out of range");
+    const DevLevel& D = S->d[level];
+    const HostLevel& H = S->h[level];
+    out->nbr = D.nbr;
+    out->bcoord = D.bcoord;
+    out->binst = D.binst;
+    out->bflags = D.bflags;
+    out->cflags = D.cflags;
+    out->c2f_gchild = D.c2f_gchild;
+    out->c2f_gcap = D.c2f_gcap;
+    out->c2f_gmask = D.c2f_gmask;
+    out->cap_slots = D.cap_slots;
+    out->f2c_ghost = D.f2c_ghost;
+    out->f2c_child = D.f2c_child;
+    out->force_links = D.force_links;
+    out->abeg = D.abeg;
+    out->lbeg = D.lbeg;
+    out->nb = H.nb;
+    out->nactive = H.nactive;
+    out->n_groups = H.n_groups;
+    out->n_cap = H.n_cap;
+    out->n_f2c = H.n_f2c;
+    out->n_links = H.n_links;
     return FL_OK;
 }
 
@@ -1476,9 +1726,9 @@ fl_status fl_lbm_stats_get(const fl_lbm_solver* S, fl_lbm_stats* out) {
         out->leaves[l] = H.nfluid;
         out->bricks[l] = H.nb;
         out->active_bricks[l] = H.nactive;
-        out->c2f[l] = H.c2f_ghost.size();
-        out->f2c[l] = H.f2c_ghost.size();
-        out->capture[l] = H.cap_slots.size();
+        out->c2f[l] = H.n_c2f;
+        out->f2c[l] = H.n_f2c;
+        out->capture[l] = H.n_cap;
         out->updates_per_step += H.nfluid << l;
         out->variant[l] = S->variant[l];
     }
@@ -1760,41 +2010,20 @@ fl_status fl_lbm_regrid_batch(fl_lbm_solver* S, const size_t* offsets, const int
     std::vector<fl_lbm_params> ps;
     for (const auto& ih : S->inst) ps.push_back(ih.p);
     std::unique_ptr<fl_lbm_solver> N;
-    fl_status r = plan_solver(ps.data(), S->ninst(), offsets, x, y, z, lv, state, N);
+    fl_status r = build_from_host(ps.data(), S->ninst(), offsets, x, y, z, lv, state, S->stream, S, N);
     if (r) return r;
-    N->stream = S->stream;
-    N->hinst = S->hinst;  // keeps the live flags
-    N->ticks = S->ticks;  // and the rounding clock (owned by S)
-    switch (S->storage) {
-        case FL_STORE_F32: r = upload_and_init<float>(N.get()); break;
-        case FL_STORE_F16: r = upload_and_init<_Float16>(N.get()); break;
-        case FL_STORE_F16C: r = upload_and_init<fp16c>(N.get()); break;
-        default: r = upload_and_init<double>(N.get()); break;
-    }
+    return adopt_layout(S, N);
+}
+
+fl_status fl_lbm_regrid_batch_device(fl_lbm_solver* S, const size_t* offsets, const int32_t* d_x, const int32_t* d_y,
+                                     const int32_t* d_z, const uint8_t* d_level, const uint8_t* d_state) {
+    FL_REQUIRE(S && offsets, "null pointer");
+    std::vector<fl_lbm_params> ps;
+    for (const auto& ih : S->inst) ps.push_back(ih.p);
+    std::unique_ptr<fl_lbm_solver> N;
+    fl_status r = build_on_device(ps.data(), S->ninst(), offsets, d_x, d_y, d_z, d_level, d_state, S->stream, S, N);
     if (r) return r;
-    switch (S->storage) {
-        case FL_STORE_F32: r = remap<float>(S, N.get()); break;
-        case FL_STORE_F16: r = remap<_Float16>(S, N.get()); break;
-        case FL_STORE_F16C: r = remap<fp16c>(S, N.get()); break;
-        default: r = remap<double>(S, N.get()); break;
-    }
-    if (r) return r;
-    // the handle keeps its identity, stream, variants and graph setting; its
-    // layout becomes the new mesh (the old one is freed with N)
-    S->drop_graphs();
-    std::swap(S->inst, N->inst);
-    std::swap(S->L, N->L);
-    for (int l = 0; l < 24; ++l) {
-        std::swap(S->h[l], N->h[l]);
-        std::swap(S->d[l], N->d[l]);
-    }
-    std::swap(S->dinst, N->dinst);
-    std::swap(S->acc, N->acc);
-    std::swap(S->allocs, N->allocs);
-    std::swap(S->device_bytes, N->device_bytes);
-    // the velocity snapshots and window sums belonged to the old layout
-    S->nsamples = 0;
-    return FL_OK;
+    return adopt_layout(S, N);
 }
 
 fl_status fl_lbm_regrid(fl_lbm_solver* S, const int32_t* x, const int32_t* y, const int32_t* z, const uint8_t* lv,

@@ -68,20 +68,42 @@ class LbmBatch:
         """params: list of LbmParams; meshes: list of dicts with x, y, z, level, state."""
         assert len(params) == len(meshes) and len(params) >= 1
         self.params = list(params)
-        self.meshes = [dict(m) for m in meshes]
+        self._meshes = [dict(m) for m in meshes]
+        self._dev = None
         off, cat = _concat(meshes)
         self._arr = (LbmParams * len(params))(*params)
         self.h = C.c_void_p()
         check(lib().fl_lbm_create_batch(self._arr, len(params), off.ctypes.data, cat["x"].ctypes.data,
                                         cat["y"].ctypes.data, cat["z"].ctypes.data, cat["level"].ctypes.data,
                                         cat["state"].ctypes.data, stream, C.byref(self.h)))
-        self.fluid = [_fluid(m) for m in meshes]
+        self._fluid = [_fluid(m) for m in meshes]
         if graph:
             self.set_graph(True)
 
     @property
     def ninst(self):
         return len(self.params)
+
+    def _download(self):
+        """Host copies of the leaves after a regrid from device leaves."""
+        cells, off = self._dev
+        h = cells.to_host()
+        self._meshes = [{k: h[k][off[i]:off[i + 1]].copy() for k in ("x", "y", "z", "level", "state")}
+                        for i in range(self.ninst)]
+        self._fluid = [_fluid(m) for m in self._meshes]
+        self._dev = None
+
+    @property
+    def meshes(self):
+        if self._meshes is None:
+            self._download()
+        return self._meshes
+
+    @property
+    def fluid(self):
+        if self._fluid is None:
+            self._download()
+        return self._fluid
 
     def step(self, n=1):
         check(lib().fl_lbm_step(self.h, int(n)))
@@ -187,8 +209,25 @@ class LbmBatch:
         off, cat = _concat(meshes)
         check(lib().fl_lbm_regrid_batch(self.h, off.ctypes.data, cat["x"].ctypes.data, cat["y"].ctypes.data,
                                         cat["z"].ctypes.data, cat["level"].ctypes.data, cat["state"].ctypes.data))
-        self.meshes = [dict(m) for m in meshes]
-        self.fluid = [_fluid(m) for m in meshes]
+        self._meshes = [dict(m) for m in meshes]
+        self._fluid = [_fluid(m) for m in meshes]
+        self._dev = None
+
+    def regrid_batch_device(self, cells, offsets):
+        """Switch every instance to new leaves held on the device (an
+        amr.GpuCells, instance k owning leaves [offsets[k], offsets[k+1])):
+        the layout is built by kernels and the populations carried over
+        without the leaves leaving the device.  Host copies of the leaves
+        are made only if a later call needs them (queries by default leaf
+        list, mesh properties), from `cells`, which the batch keeps
+        referenced: read `meshes` first if `cells` is to be freed or
+        overwritten."""
+        off = np.ascontiguousarray(offsets, np.uint64)
+        assert len(off) == self.ninst + 1 and int(off[-1]) <= cells.n
+        check(lib().fl_lbm_regrid_batch_device(self.h, off.ctypes.data, cells.x.p, cells.y.p, cells.z.p,
+                                               cells.level.p, cells.state.p))
+        self._meshes = self._fluid = None
+        self._dev = (cells, [int(v) for v in off])
 
     def close(self):
         if self.h:
@@ -271,3 +310,7 @@ class LbmSolver(LbmBatch):
 
     def regrid(self, x, y, z, level, state):
         self.regrid_batch([dict(x=x, y=y, z=z, level=level, state=state)])
+
+    def regrid_device(self, cells):
+        """regrid from the n leaves of an amr.GpuCells on the device."""
+        self.regrid_batch_device(cells, [0, cells.n])

@@ -59,6 +59,15 @@ fl_status fl_stream_create(fl_stream* s);
 fl_status fl_stream_destroy(fl_stream s);
 fl_status fl_stream_sync(fl_stream s);
 fl_status fl_device_sync(void);
+/* Device block cache.  Solver tables, population buffers and the layout
+ * builders' temporaries come from a per-device cache that keeps freed
+ * blocks (at most an eighth of the device's memory, 32 GiB at most) for
+ * later requests, so regrids do not pay hipMalloc / hipFree per table.
+ * fl_release_cached_memory returns every cached block to HIP (for callers
+ * sharing the device with other allocators); fl_cached_memory_bytes reports
+ * the bytes held.  No counterpart in the reference. */
+fl_status fl_release_cached_memory(void);
+fl_status fl_cached_memory_bytes(uint64_t* bytes);
 fl_status fl_event_create(fl_event* e);
 fl_status fl_event_destroy(fl_event e);
 fl_status fl_event_record(fl_event e, fl_stream s);
@@ -303,6 +312,41 @@ fl_status fl_lbm_create_batch(const fl_lbm_params* params, int ninst, const size
                               const int32_t* h_x, const int32_t* h_y, const int32_t* h_z,
                               const uint8_t* h_level, const uint8_t* h_state, fl_stream s,
                               fl_lbm_solver** out);
+/* Leaves already on the device (for instance the output of fl_regrid): the
+ * layout -- bricks, neighbour rows, link words, ghost lists, force links --
+ * is built by kernels and no leaf travels through the host.  fl_lbm_create*
+ * upload host leaves once and take this path.  (The reference builds its
+ * index tables on the host: MergeEngine.cpp:90-138, BalanceEnforcer.cpp:
+ * 100-143.) */
+fl_status fl_lbm_create_batch_device(const fl_lbm_params* params, int ninst, const size_t* offsets,
+                                     const int32_t* d_x, const int32_t* d_y, const int32_t* d_z,
+                                     const uint8_t* d_level, const uint8_t* d_state, fl_stream s,
+                                     fl_lbm_solver** out);
+/* The layout of one level as device pointers and sizes (read-only views
+ * owned by the solver): the tables both builders produce.  No counterpart
+ * in the reference. */
+typedef struct {
+    const void* nbr;          /* int32 [nb][27] */
+    const void* bcoord;       /* int32 [nb][3] */
+    const void* binst;        /* uint16 [nb] */
+    const void* bflags;       /* uint32 [nb] */
+    const void* cflags;       /* uint64 [nb * 64] */
+    const void* c2f_gchild;   /* uint32 [n_groups][8] */
+    const void* c2f_gcap;     /* uint32 [n_groups][7] */
+    const void* c2f_gmask;    /* uint32 [n_groups][8] */
+    const void* cap_slots;    /* uint32 [n_cap] */
+    const void* f2c_ghost;    /* uint32 [n_f2c] */
+    const void* f2c_child;    /* uint32 [n_f2c][8] */
+    const void* force_links;  /* uint32 [n_links] */
+    const void* abeg;         /* uint32 [ninst + 1] */
+    const void* lbeg;         /* uint32 [ninst + 1] */
+    uint64_t nb, nactive, n_groups, n_cap, n_f2c, n_links;
+} fl_lbm_layout;
+fl_status fl_lbm_layout_get(const fl_lbm_solver* solver, int level, fl_lbm_layout* out);
+/* Process-wide: on != 0 makes fl_lbm_create* and fl_lbm_regrid* build the
+ * layout on the host instead (the reference implementation the device
+ * builder is tested against; fl_lbm_plan_bytes always uses it). */
+fl_status fl_lbm_layout_on_host(int on);
 /* Device bytes fl_lbm_create_batch would allocate for these leaves (host
  * layout only; no device work) -- the admission footprint of a request. */
 fl_status fl_lbm_plan_bytes(const fl_lbm_params* params, int ninst, const size_t* offsets,
@@ -354,6 +398,11 @@ fl_status fl_lbm_regrid(fl_lbm_solver* solver, const int32_t* h_x, const int32_t
 fl_status fl_lbm_regrid_batch(fl_lbm_solver* solver, const size_t* offsets, const int32_t* h_x,
                               const int32_t* h_y, const int32_t* h_z, const uint8_t* h_level,
                               const uint8_t* h_state);
+/* The same with the new leaves on the device (no leaf travels through the
+ * host; the population transfer lists are built on the device too). */
+fl_status fl_lbm_regrid_batch_device(fl_lbm_solver* solver, const size_t* offsets, const int32_t* d_x,
+                                     const int32_t* d_y, const int32_t* d_z, const uint8_t* d_level,
+                                     const uint8_t* d_state);
 /* FP64 device reductions (synchronous).  Per instance: per-brick wave sums,
  * then one workgroup per instance over its brick range in a fixed order.
  * fl_lbm_totals / fl_lbm_force sum the instances. */
