@@ -21,8 +21,8 @@
 
 namespace fl {
 fl_status hash_clear_impl(uint64_t* tk, uint32_t* tv, uint64_t cap, hipStream_t s);
-fl_status cellmap_build_impl(uint64_t* tk, uint32_t* tv, uint64_t cap, const uint64_t* keys, size_t n,
-                             hipStream_t s);
+fl_status cellkey_map_build_impl(uint64_t* tk, uint32_t* tv, uint64_t cap, const uint64_t* keys, size_t n,
+                                 hipStream_t s);
 fl_status hash_build_impl(uint64_t* tk, uint32_t* tv, uint64_t cap, const uint64_t* keys,
                           const uint32_t* vals, size_t n, uint32_t* probes, hipStream_t s);
 
@@ -67,7 +67,7 @@ __device__ inline uint32_t find_cell(const CellHash& h, int l, int32_t x, int32_
     if (l < 0 || l > 19 || x < 0 || y < 0 || z < 0 || x >= kKeyCoordLimit || y >= kKeyCoordLimit ||
         z >= kKeyCoordLimit)
         return kInvalid;
-    return cell_lookup(h.tk, h.tv, h.cap, cell_hkey((uint32_t)x, (uint32_t)y, (uint32_t)z, l));
+    return cellkey_lookup(h.tk, h.tv, h.cap, cell_hkey((uint32_t)x, (uint32_t)y, (uint32_t)z, l));
 }
 
 // The key find_cell looks up, or EMPTY (absent) outside the key's range.
@@ -117,13 +117,13 @@ fl_status build_cell_hash(CellsC c, size_t n, uint64_t* keys, uint64_t* tk, uint
     FL_CHECK_LAUNCH();
     fl_status st = hash_clear_impl(tk, tv, cap, s);
     if (st) return st;
-    return cellmap_build_impl(tk, tv, cap, keys, n, s);
+    return cellkey_map_build_impl(tk, tv, cap, keys, n, s);
 }
 
 // ---------------------------------------------------------------- a9
 // Per face: the first cell holding the face point at levels L, L-1, .., 0;
 // else kFiner when a level L+1 cell touches it; else kInvalid.  The six
-// faces are searched in lockstep, level by level (cell_lookup_n), so a
+// faces are searched in lockstep, level by level (cellkey_lookup_n), so a
 // cell's lookups overlap; each face's result is the same first hit.
 __global__ __launch_bounds__(kBlock) void k_face_nbr(CellsC c, size_t n, const Root root, CellHash h,
                                                      uint32_t* __restrict__ nbr) {
@@ -148,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_face_nbr(CellsC c, size_t n, const R
 #pragma unroll
             for (int f = 0; f < 6; ++f)
                 key[f] = ((pend >> f) & 1u) ? cell_key_or_empty(l, px[f] >> sh, py[f] >> sh, pz[f] >> sh) : kEmptyKey;
-            cell_lookup_n<6>(h.tk, h.tv, h.cap, key, got);
+            cellkey_lookup_n<6>(h.tk, h.tv, h.cap, key, got);
 #pragma unroll
             for (int f = 0; f < 6; ++f)
                 if (((pend >> f) & 1u) && got[f] != kInvalid) {
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_face_nbr(CellsC c, size_t n, const R
             for (int q = 0; q < 8; ++q)
                 key[q] = cell_key_or_empty(L + 1, 2 * px[f] + (q & 1), 2 * py[f] + ((q >> 1) & 1),
                                            2 * pz[f] + ((q >> 2) & 1));
-            cell_lookup_n<8>(h.tk, h.tv, h.cap, key, got);
+            cellkey_lookup_n<8, false>(h.tk, h.tv, h.cap, key, got);
 #pragma unroll
             for (int q = 0; q < 8; ++q)
                 if (got[q] != kInvalid) r[f] = kFiner;
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void k_detect0(CellsC c, size_t n, int M,
             const int32_t ax = c.x[i] << (M - L), ay = c.y[i] << (M - L), az = c.z[i] << (M - L);
             const int32_t size = s <= M ? (1 << (M - s)) : 0;
             // per face point: the first level l = 0..M holding it decides; the
-            // six points are probed in lockstep (cell_lookup_n), which gives
+            // six points are probed in lockstep (cellkey_lookup_n), which gives
             // the same first hits, and the cell violates if any face does
             int32_t px[6], py[6], pz[6];
 #pragma unroll
@@ -239,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_detect0(CellsC c, size_t n, int M,
                     key[t] = ((pend >> t) & 1u) && in ? cell_hkey((uint32_t)qx, (uint32_t)qy, (uint32_t)qz, l)
                                                       : kEmptyKey;
                 }
-                cell_lookup_n<6>(h.tk, h.tv, h.cap, key, got);
+                cellkey_lookup_n<6, false>(h.tk, h.tv, h.cap, key, got);
 #pragma unroll
                 for (int t = 0; t < 6; ++t)
                     if (((pend >> t) & 1u) && got[t] != kInvalid) {
@@ -299,7 +299,7 @@ __device__ bool detect_mode1(const CellHash& h, const uint8_t* __restrict__ shad
                 key[k] = cell_key_or_empty(L, px, py, pz);
             }
         }
-        cell_lookup_n<6>(h.tk, h.tv, h.cap, key, got);
+        cellkey_lookup_n<6, false>(h.tk, h.tv, h.cap, key, got);
 #pragma unroll
         for (int k = 0; k < 6; ++k)
             if (got[k] != kInvalid) open &= ~(1u << (d0 + k));
@@ -671,7 +671,10 @@ inline size_t al(size_t v) { return (v + 255) & ~size_t(255); }
 
 RegridLayout regrid_layout(size_t n) {
     RegridLayout L;
-    L.cap = fl_hash_capacity(n);
+    // twice the public table's capacity (load <= 0.3): the map's 2x2x2 block
+    // groups are displaced as units, and at half the load a wave's lookups
+    // finish in about two group rounds instead of four
+    L.cap = std::min<uint64_t>(2 * fl_hash_capacity(n), 1ull << 31);
     size_t o = 0;
     L.keys = o; o = al(o + n * 8);
     L.tk = o; o = al(o + L.cap * 8);
@@ -735,7 +738,7 @@ fl_status run_balance(const fl_regrid_config* cfg, fl_cells* cells, size_t n, co
             FL_CHECK_LAUNCH();
             fl_status st = hash_clear_impl(tk, tv, x.L.cap, x.s);
             if (st) return st;
-            st = cellmap_build_impl(tk, tv, x.L.cap, keys, n, x.s);
+            st = cellkey_map_build_impl(tk, tv, x.L.cap, keys, n, x.s);
             if (st) return st;
             hipLaunchKernelGGL(k_detect0, dim3(sgrid(n)), dim3(kBlock), 0, x.s, c, n, cfg->anchor_level, shadow,
                                h, viol, cnt);
@@ -948,8 +951,13 @@ fl_status fl_regrid(const fl_regrid_config* cfg, fl_cells* in, size_t n, fl_cell
     FL_CHECK_LAUNCH();
     if (n > 0) {
         CellsC c = cview(in);
-        r = build_cell_hash(c, n, x.at<uint64_t>(L.keys), x.at<uint64_t>(L.tk), x.at<uint32_t>(L.tv), L.cap, st);
-        if (r) return r;
+        // a D3Q19-rule balance (mode 1) has built this map over the same
+        // cells and changed only their flags; the reference rule's map holds
+        // anchor keys and is replaced
+        if (cfg->balance_mode <= 0) {
+            r = build_cell_hash(c, n, x.at<uint64_t>(L.keys), x.at<uint64_t>(L.tk), x.at<uint32_t>(L.tv), L.cap, st);
+            if (r) return r;
+        }
         CellHash h{x.at<uint64_t>(L.tk), x.at<uint32_t>(L.tv), L.cap};
         uint32_t* gid = x.at<uint32_t>(L.gid);
         uint32_t* leaders = x.at<uint32_t>(L.leaders);

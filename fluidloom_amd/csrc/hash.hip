@@ -40,11 +40,16 @@ __global__ __launch_bounds__(kBlock) void k_query(const uint64_t* __restrict__ t
     }
 }
 
+template <bool kCellKeys>
 __global__ __launch_bounds__(kBlock) void k_cell_build(uint64_t* __restrict__ tk, uint32_t* __restrict__ tv,
                                                        uint64_t cap, const uint64_t* __restrict__ keys, size_t n) {
     for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
         const uint64_t k = keys[i];
-        if (k != kEmptyKey) cell_insert(tk, tv, cap, k, (uint32_t)i);
+        if (k == kEmptyKey) continue;
+        if (kCellKeys)
+            cellkey_insert(tk, tv, cap, k, (uint32_t)i);
+        else
+            cell_insert(tk, tv, cap, k, (uint32_t)i);
     }
 }
 
@@ -71,18 +76,31 @@ fl_status hash_build_impl(uint64_t* tk, uint32_t* tv, uint64_t cap, const uint64
     return FL_OK;
 }
 
-// exact cell map (value = smallest index holding the key) for the AMR kernels.
-// Its lookups stop at the first empty slot, so the load must stay <= 0.6:
-// with the capacity rule's 2^30 ceiling that bounds n by kMaxMapCells (which
-// also keeps cell indices inside u32 below the kFiner / kInvalid markers).
-fl_status cellmap_build_impl(uint64_t* tk, uint32_t* tv, uint64_t cap, const uint64_t* keys, size_t n,
-                             hipStream_t s) {
+// Exact maps (value = smallest index holding the key).  Their lookups stop at
+// the first empty slot, so the load must stay <= 0.6: with the capacity
+// rule's 2^30 ceiling that bounds n by kMaxMapCells (which also keeps
+// indices inside u32 below the kFiner / kInvalid markers).
+template <bool kCellKeys>
+fl_status map_build(uint64_t* tk, uint32_t* tv, uint64_t cap, const uint64_t* keys, size_t n, hipStream_t s) {
     if (n == 0) return FL_OK;
     FL_REQUIRE(n <= kMaxMapCells, "too many cells for the cell map (at most 644,245,094)");
     FL_REQUIRE((double)n <= 0.6 * (double)cap, "cell map capacity below n / 0.6");
-    hipLaunchKernelGGL(k_cell_build, dim3(cap_grid(n)), dim3(kBlock), 0, s, tk, tv, cap, keys, n);
+    FL_REQUIRE(cap >= 8 && (reinterpret_cast<uintptr_t>(tk) & 15u) == 0, "cell map keys must be 16 B aligned");
+    hipLaunchKernelGGL(k_cell_build<kCellKeys>, dim3(cap_grid(n)), dim3(kBlock), 0, s, tk, tv, cap, keys, n);
     FL_CHECK_LAUNCH();
     return FL_OK;
+}
+
+// keys of any packing, per-key slots (the lattice layout's brick map)
+fl_status cellmap_build_impl(uint64_t* tk, uint32_t* tv, uint64_t cap, const uint64_t* keys, size_t n,
+                             hipStream_t s) {
+    return map_build<false>(tk, tv, cap, keys, n, s);
+}
+
+// packed AMR cell keys, 2x2x2-block groups (the AMR kernels' cell map)
+fl_status cellkey_map_build_impl(uint64_t* tk, uint32_t* tv, uint64_t cap, const uint64_t* keys, size_t n,
+                                 hipStream_t s) {
+    return map_build<true>(tk, tv, cap, keys, n, s);
 }
 
 }  // namespace fl

@@ -153,4 +153,105 @@ __device__ inline void cell_lookup_n(const uint64_t* __restrict__ tkeys, const u
     }
 }
 
+// Cell-key map for the AMR kernels (packed keys L<<57 | x<<38 | y<<19 | z),
+// exact like the map above but laid out for spatial locality: the eight
+// cells of one 2x2x2 block (same level, coordinates equal but for their
+// lowest bits) hash to one aligned group of eight slots, each at the offset
+// of its octant.  A cell's D3Q19 neighbours then lie in at most 8 groups (6
+// of the 18 in its own), each 64 B of keys and 32 B of values, and the cells
+// of one refined region share those lines, where a per-cell hash sends every
+// lookup to two random lines.  Full blocks collide as units, so a displaced
+// key sits several slots past its home (about 5 on average at load 0.57):
+// lookups therefore scan whole groups, the eight keys of a group in one
+// round (four 16 B loads), keeping the linear-probing order -- slots from
+// the home offset to the end of its group, then whole groups -- so a round
+// count is the group displacement (1.7 on average at load 0.57).  Keys whose
+// low bits are all zero (coarse anchors) fill a group from its first slot:
+// bucketed linear probing at the same load.  The key arrays must be 16 B
+// aligned.
+__device__ inline uint64_t cellkey_slot(uint64_t key, uint64_t cap) {
+    constexpr uint64_t kLow = (1ull << 38) | (1ull << 19) | 1ull;
+    const int bits = 63 - __clzll((long long)cap);  // cap >= 8
+    const uint64_t group = ((key & ~kLow) * 0x9e3779b97f4a7c15ull) >> (64 - bits);
+    const uint64_t oct = (key & 1ull) | ((key >> 18) & 2ull) | ((key >> 36) & 4ull);
+    return (group & ~7ull) | oct;
+}
+
+// Bit t set where group slot t (of the eight at g) holds `key`, in `hit`, or
+// is empty, in `emp`.
+__device__ inline void cellkey_scan(const uint64_t* __restrict__ tkeys, uint64_t g, uint64_t key, unsigned& hit,
+                                    unsigned& emp) {
+    const ulonglong2* p = reinterpret_cast<const ulonglong2*>(tkeys + g);
+    hit = emp = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const ulonglong2 v = p[q];
+        hit |= (v.x == key ? 1u : 0u) << (2 * q) | (v.y == key ? 1u : 0u) << (2 * q + 1);
+        emp |= (v.x == 0xFFFFFFFFFFFFFFFFull ? 1u : 0u) << (2 * q) |
+               (v.y == 0xFFFFFFFFFFFFFFFFull ? 1u : 0u) << (2 * q + 1);
+    }
+}
+
+// Insertion probes slot by slot with claim_slot (coherent loads, a CAS only
+// on an empty slot): the eight cells of a displaced block usually come from
+// one wave and would race for the same free slots under a group scan.  The
+// table is the same linear-probing table either way.
+__device__ inline void cellkey_insert(uint64_t* tkeys, uint32_t* tvals, uint64_t cap, uint64_t key, uint32_t val) {
+    uint64_t s = cellkey_slot(key, cap);
+    for (uint64_t p = 0; p < cap; ++p) {
+        if (claim_slot(tkeys, s, key)) {
+            keep_min(tvals, s, val);
+            return;
+        }
+        s = (s + 1) & (cap - 1);
+    }
+}
+
+// N lookups advanced together, one group per lookup per round.  out[j]: the
+// stored value (kValues) or 0 (presence only), 0xFFFFFFFF when key[j] is
+// absent or EMPTY.  Values are loaded after the probe rounds, all at once.
+template <int N, bool kValues = true>
+__device__ inline void cellkey_lookup_n(const uint64_t* __restrict__ tkeys, const uint32_t* __restrict__ tvals,
+                                        uint64_t cap, const uint64_t (&key)[N], uint32_t (&out)[N]) {
+    uint64_t s[N];
+    unsigned open = 0, found = 0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        out[j] = 0xFFFFFFFFu;
+        s[j] = cellkey_slot(key[j], cap);
+        if (key[j] != 0xFFFFFFFFFFFFFFFFull) open |= 1u << j;
+    }
+    for (uint64_t r = 0; r < cap && open; r += 8) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            if (!((open >> j) & 1u)) continue;
+            const uint64_t g = s[j] & ~7ull;
+            unsigned hit, emp;
+            cellkey_scan(tkeys, g, key[j], hit, emp);
+            const unsigned m = (hit | emp) & (0xFFu << (s[j] & 7ull));
+            if (!m) {
+                s[j] = (g + 8) & (cap - 1);
+                continue;
+            }
+            const unsigned t = (unsigned)__builtin_ctz(m);
+            open &= ~(1u << j);
+            if ((hit >> t) & 1u) {
+                found |= 1u << j;
+                s[j] = g + t;
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+        if ((found >> j) & 1u) out[j] = kValues ? tvals[s[j]] : 0u;
+}
+
+__device__ inline uint32_t cellkey_lookup(const uint64_t* __restrict__ tkeys, const uint32_t* __restrict__ tvals,
+                                          uint64_t cap, uint64_t key) {
+    const uint64_t k[1] = {key};
+    uint32_t o[1];
+    cellkey_lookup_n<1>(tkeys, tvals, cap, k, o);
+    return o[0];
+}
+
 }  // namespace fl
