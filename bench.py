@@ -26,6 +26,13 @@ committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this command;
 `cpu_baseline` is the CPU port of the lattice step in FP32 (the oracle's
 FP32 build, OpenMP) on a bounded quarter-size instance of the same
 configuration.
+
+`amr_integer` (after the timed loop; --no-amr skips it) times the AMR integer
+path on the device -- the 64M-cell micro-benchmark's stages (Hilbert and
+rot/dir keys, the 64-bit sort with the digit passes it ran, scan, regrid
+pass, face-neighbour rebuild, hash build / query) and a cascading D3Q19
+balance on this sphere mesh -- with the oracle's single-threaded C
+restatement of each stage timed beside it on bounded samples.
 """
 from __future__ import annotations
 
@@ -88,6 +95,84 @@ def cpu_baseline(budget_s=15.0):
                       f"{nx0}x{ny0}x{nz0} r={r:g} 3 levels ({upd} leaf updates/step), {steps} steps in {dt:.1f} s"}
 
 
+def cpu_amr_baseline():
+    """The AMR integer stages on the host: the oracle's C restatement
+    (single-threaded) on bounded samples of the 64M-cell workload's
+    generator -- 4M cells for the streaming stages, 1M for the face
+    neighbours, 256K for the regrid pass, whose reference-rule balance keeps
+    every holder of an anchor in one probe chain (as the reference's index
+    table does) and grows faster than linearly on these seeded cells."""
+    from oracle import oracle as O
+
+    def timed(f):
+        t0 = time.perf_counter()
+        r = f()
+        return r, time.perf_counter() - t0
+
+    out = {"cores": 1, "kind": "port", "unit": "M cells/s"}
+    rates = {}
+    n = 1 << 22
+    x, y, z, lv = O.gen_cells(42, n, 10)
+    _, dt = timed(lambda: O.rotdir_encode(x, y, z, 8))
+    rates["rotdir"] = n / dt / 1e6
+    k, dt = timed(lambda: O.key63(x, y, z, lv))
+    rates["keygen"] = n / dt / 1e6
+    _, dt = timed(lambda: O.sort_pairs(k, np.arange(n, dtype=np.uint32)))
+    rates["sort"] = n / dt / 1e6
+    _, dt = timed(lambda: O.scan_exclusive(np.arange(n, dtype=np.uint32)))
+    rates["scan"] = n / dt / 1e6
+    rk = np.random.default_rng(42).integers(0, 1 << 63, n, dtype=np.uint64)
+    cap = O.hash_capacity(n)
+    (tk, tv, _), dt = timed(lambda: O.hash_build(rk, np.arange(n, dtype=np.uint32), cap))
+    rates["hash_build"] = n / dt / 1e6
+    _, dt = timed(lambda: O.hash_query(tk, tv, cap, rk))
+    rates["hash_query"] = n / dt / 1e6
+    m = 1 << 20
+    c = O.CellSet(x=x[:m], y=y[:m], z=z[:m], level=lv[:m])
+    _, dt = timed(lambda: O.face_neighbors(c))
+    rates["neighbors"] = m / dt / 1e6
+    r = 1 << 18
+    c = O.gen_flags(O.CellSet(x=x[:r], y=y[:r], z=z[:r], level=lv[:r]), 42, 0.05, 0.03)
+    _, dt = timed(lambda: O.regrid(c, 0, 10, 10))
+    rates["regrid"] = r / dt / 1e6
+    out["rates"] = {k2: round(v, 2) for k2, v in rates.items()}
+    out["sample"] = (f"oracle C restatement, 1 thread: {n} seeded cells (levels 0-10, 2^10 root) for the codecs, "
+                     f"sort, scan and hash; {m} for the face neighbours; {r} for the regrid pass")
+    return out
+
+
+def amr_stages(mesh, ctr, r, root, log):
+    """The AMR integer path on the device: the 64M-cell micro-benchmark's
+    stages (best of two runs) and a cascading D3Q19 balance on this sphere
+    mesh."""
+    from fluidloom_amd import amr, amr_bench
+    runs = [amr_bench.run() for _ in range(2)]  # the first also loads each kernel's code
+    res = dict(runs[-1])
+    for k in res:
+        if k.endswith("_ms"):
+            res[k] = min(r[k] for r in runs)
+    res["sort_gbs"] = max(r["sort_gbs"] for r in runs)
+    res["scan_gbs"] = max(r["scan_gbs"] for r in runs)
+    cells = amr.GpuCells.from_host(**{k: mesh[k] for k in ("x", "y", "z", "level", "state")})
+    casc = amr_bench.cascade(cells, ctr, 2 * r, 4, root)
+    cells.free()
+    log(f"amr stages: {res}; cascade {casc}")
+    keep = ("rotdir_ms", "keygen_ms", "sort_ms", "sort_passes", "sort_gbs", "scan_ms", "scan_gbs", "regrid_ms",
+            "cells_out", "neighbors_ms", "hash_build_ms", "hash_query_ms")
+    out = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in res.items() if k in keep}
+    out["rates_mcells_s"] = {
+        "rotdir": round(res["cells"] / res["rotdir_ms"] / 1e3, 1),
+        "keygen": round(res["cells"] / res["keygen_ms"] / 1e3, 1),
+        "sort": round(res["cells"] / res["sort_ms"] / 1e3, 1),
+        "scan": round(res["cells"] / res["scan_ms"] / 1e3, 1),
+        "hash_build": round(res["cells"] / res["hash_build_ms"] / 1e3, 1),
+        "hash_query": round(res["cells"] / res["hash_query_ms"] / 1e3, 1),
+        "neighbors": round(res["cells_out"] / res["neighbors_ms"] / 1e3, 1),
+        "regrid": round(res["cells"] / res["regrid_ms"] / 1e3, 1)}
+    out["cascade_balance"] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in casc.items()}
+    return out
+
+
 def copy_bandwidth(_lib, L, stream, nbytes=2 << 30, iters=10):
     C = _lib.C
     a, b = _lib.DeviceArray(nbytes, np.uint8), _lib.DeviceArray(nbytes, np.uint8)
@@ -115,6 +200,7 @@ def main():
     ap.add_argument("--storage", default="f32", choices=["f32", "f16", "f16c", "f64"])
     ap.add_argument("--scale", type=int, default=1, help="divide the base grid and radii by this")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-amr", action="store_true", help="skip the AMR integer stages (64M cells, cascade)")
     ap.add_argument("--regrid-every", type=int, default=100)
     ap.add_argument("--regrid-keep-layout", action="store_true",
                     help="keep the solver's layout when the regrid pass changed no leaf")
@@ -268,10 +354,17 @@ def main():
     copy_gbs = copy_bandwidth(_lib, L, stream)
     log(f"plain copy kernel: {copy_gbs:.0f} GB/s")
 
+    amr_out = None
+    if not args.no_amr:
+        log("AMR integer stages (64M seeded cells; cascading balance on the sphere mesh)...")
+        amr_out = amr_stages(mesh, ctr, r, (nx0, ny0, nz0), log)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, bounded sample)...")
         cpu = cpu_baseline()
+        if amr_out is not None:
+            amr_out["cpu_baseline"] = cpu_amr_baseline()
 
     out = {
         "metric": METRIC,
@@ -312,6 +405,7 @@ def main():
                      "kernel": f"k_collide {finest_variant} level {levels} ({leaves[levels]} leaves x {bpu} B)"},
         "copy_kernel_gbs": round(copy_gbs, 1),
         "finest_variant_ms": variant_ms,
+        "amr_integer": amr_out,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

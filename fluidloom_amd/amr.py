@@ -137,6 +137,26 @@ def regrid(cells: GpuCells, balance_mode=1, anchor_level=10, max_split_level=8, 
     return out, st
 
 
+def balance(cells: GpuCells, balance_mode=1, anchor_level=10, max_split_level=8, max_iters=64, root=(1, 1, 1),
+            temp=None, stream=None) -> RegridStats:
+    """2:1 balance to a fixpoint on the device (fl_balance): marks split
+    flags in place and returns the iteration counters (the cell count is
+    unchanged)."""
+    n = cells.n
+    cfg = RegridConfig()
+    cfg.balance_mode, cfg.anchor_level, cfg.max_split_level = balance_mode, anchor_level, max_split_level
+    cfg.max_balance_iters = max_iters
+    for a in range(3):
+        cfg.root[a] = int(root[a])
+    tb = lib().fl_regrid_temp_bytes(n)
+    tmp = temp if temp is not None and temp.nbytes >= tb else DeviceArray(max(tb, 1), np.uint8)
+    res = RegridResult()
+    cs = cells.struct()
+    check(lib().fl_balance(C.byref(cfg), C.byref(cs), n, tmp.p, tb, C.byref(res), stream))
+    return RegridStats(n, 0, 0, int(res.balance_iterations), int(res.balance_violations),
+                       int(res.balance_marked), bool(res.balance_converged))
+
+
 def mark_ball(cells: GpuCells, center, radius, target_level, stream=None) -> int:
     m = C.c_uint64(0)
     cs = cells.struct()

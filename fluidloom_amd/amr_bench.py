@@ -4,7 +4,14 @@ complete sibling groups; the reference's rot/dir keys, Hilbert keys, 64-bit
 key sort, a 64M-element exclusive scan, one regrid pass
 (reference balance rule to fixpoint, split, merge, scan compaction) and the
 hash face-neighbour rebuild.  Each stage is timed with HIP events; GB/s is
-computed from the algorithmic bytes of that stage.
+computed from the algorithmic bytes of that stage (for the sort: the digit
+passes it actually ran).
+
+The seeded cells overlap (they are not an octree), and the reference balance
+rule converges on them in one pass, so cascade() times a balance that does
+cascade: the D3Q19 rule (mode 1) to its fixpoint on a valid graded octree
+(the 3-level sphere mesh, 76M leaves) after every fluid leaf of its finest
+region is flagged for one more level.
 """
 from __future__ import annotations
 
@@ -69,9 +76,16 @@ def run(n=64 << 20, seed=42, p_split=0.05, p_merge=0.03, max_level=10, log=None)
         check(L.fl_sort_pairs_u64u32(keys.p, vals.p, ka.p, va.p, n, 0, 64, st.p, stb, None))
     out["sort_ms"] = t.ms
     out["sort_gkeys_s"] = n / t.ms / 1e6
-    # 8 one-byte digit passes over u64 key + u32 value (12 B read + 12 B written
-    # per pass) plus the upfront histogram read of the keys (8 B)
-    out["sort_gbs"] = (8 + 8 * 24) * n / t.ms / 1e6
+    # the sort skips a one-byte digit that every key shares (found by its
+    # upfront histogram): count the digits that vary over these keys
+    kh = keys.to_host()
+    var = int(np.bitwise_or.reduce(kh ^ kh[0])) if n else 0
+    del kh
+    passes = sum(1 for d in range(8) if (var >> (8 * d)) & 0xFF)
+    out["sort_passes"] = passes
+    # bytes moved: the upfront histogram reads the keys (8 B), each pass reads
+    # and writes u64 key + u32 value (24 B)
+    out["sort_gbs"] = (8 + passes * 24) * n / t.ms / 1e6
     del ka, va, st
     # exclusive scan of n u32 (4 B read + 4 B written per element)
     sin, sout, stot = DeviceArray(n, np.uint32), DeviceArray(n, np.uint32), DeviceArray(1, np.uint32)
@@ -141,6 +155,39 @@ def run(n=64 << 20, seed=42, p_split=0.05, p_merge=0.03, max_level=10, log=None)
     return out
 
 
+def cascade(cells: amr.GpuCells, center, radius, target_level, root, temp=None):
+    """Flags every fluid leaf below target_level whose cell lies within
+    `radius` of `center`, then runs the D3Q19 2:1 balance (mode 1) to its
+    fixpoint on the device.  `cells` is modified (flags).  Returns the flag
+    count and the balance's counters and time (HIP events around the whole
+    fl_balance call, which reads its counters back after each iteration)."""
+    flagged = amr.mark_ball(cells, center, radius, target_level)
+    if temp is None:
+        temp = amr.regrid_temp(cells.n)
+    t = _Timer()
+    w0 = time.perf_counter()
+    with t:
+        st = amr.balance(cells, balance_mode=1, max_split_level=target_level, root=root, temp=temp)
+    wall = (time.perf_counter() - w0) * 1e3
+    return {"cells": cells.n, "flagged": flagged, "iterations": st.balance_iterations,
+            "violations": st.balance_violations, "marked": st.balance_marked, "converged": st.balance_converged,
+            "ms": t.ms, "wall_ms": wall, "mcells_iter_s": cells.n * st.balance_iterations / t.ms / 1e3}
+
+
+def sphere_cascade(scale=1, log=None):
+    """cascade() on the 3-level sphere mesh (base 256x128x128 / scale, r = 16 /
+    scale): the whole finest region (radius 2r) is flagged for a fourth
+    level, so the balance must refine a ring of each coarser level in turn."""
+    from . import meshes
+    nx0, ny0, nz0, r = 256 // scale, 128 // scale, 128 // scale, 16.0 / scale
+    ctr = (64.0 / scale,) * 3
+    m = meshes.sphere(nx0, ny0, nz0, ctr, r, 2 * r, 3, log=log)
+    cells = amr.GpuCells.from_host(**{k: m[k] for k in ("x", "y", "z", "level", "state")})
+    out = cascade(cells, ctr, 2 * r, 4, (nx0, ny0, nz0))
+    cells.free()
+    return out
+
+
 def main():
     import argparse
     import json
@@ -149,8 +196,18 @@ def main():
     ap.add_argument("--cells", type=int, default=64 << 20)
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--repeat", type=int, default=3)
+    ap.add_argument("--cascade-scale", type=int, default=1, help="sphere scale of the cascade stage (0: skip)")
     a = ap.parse_args()
     check(lib().fl_init(0))
+    # one plain copy of 1 GiB: the reference point that scales counter runs
+    # (FETCH_SIZE / WRITE_SIZE) of this command to bytes
+    src, dst = DeviceArray(1 << 30, np.uint8), DeviceArray(1 << 30, np.uint8)
+    check(lib().fl_copy_kernel(dst.ptr, src.ptr, 1 << 30, None))
+    check(lib().fl_device_sync())
+    del src, dst
+    if a.cascade_scale:
+        casc = sphere_cascade(a.cascade_scale)
+        print(f"[amr_bench] cascade {casc}", file=sys.stderr, flush=True)
     runs = []
     for _ in range(a.repeat):
         runs.append(run(a.cells, a.seed))
@@ -159,6 +216,8 @@ def main():
     for k in list(best):
         if k.endswith("_gbs") or k.endswith("_s"):
             best[k] = max(r[k] for r in runs)
+    if a.cascade_scale:
+        best["cascade"] = casc
     print(json.dumps(best), flush=True)
 
 

@@ -179,7 +179,8 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
 // different XCDs); 1: XCD-aware remap, each XCD one contiguous eighth of the
 // groups; 2: persistent -- a grid sized to the resident capacity, each
 // workgroup walking a contiguous chunk of groups (chunks of one XCD
-// adjacent); 3 / 4: XCD-aware in chunks of 64 / 128 bricks (xcd_chunk_remap).
+// adjacent); 3 / 4 / 5 / 6: XCD-aware in chunks of 64 / 128 / 256 / 512
+// bricks (xcd_chunk_remap).
 // On the 3-level sphere's finest level (980k bricks) the chunked forms are
 // the fastest: mode 1 keeps neighbours in one L2 but streams eight distant
 // windows, mode 0 streams one window but splits neighbours over all L2s.
@@ -194,8 +195,9 @@ __global__ __launch_bounds__(64 * BPW) void k_collide(LevelDev L, const T* __res
         const unsigned g1 = g0 + per < ngroups ? g0 + per : ngroups;
         for (unsigned g = g0; g < g1; ++g) collide_brick<T>(L, A, B, (int)(g * BPW) + wave);
     } else {
+        constexpr unsigned kChunk = MODE == 3 ? 64u : MODE == 4 ? 128u : MODE == 5 ? 256u : 512u;
         const unsigned g = MODE == 1   ? xcd_remap(blockIdx.x, gridDim.x)
-                           : MODE >= 3 ? xcd_chunk_remap(blockIdx.x, gridDim.x, (MODE == 3 ? 64u : 128u) / BPW)
+                           : MODE >= 3 ? xcd_chunk_remap(blockIdx.x, gridDim.x, kChunk / BPW)
                                        : blockIdx.x;
         collide_brick<T>(L, A, B, (int)(g * BPW) + wave);
     }
@@ -871,13 +873,14 @@ hipError_t collide_variant(const LevelDev& L, const T* A, T* B, hipStream_t s) {
 
 }  // namespace
 
-int collide_variants() { return 17; }
+int collide_variants() { return 21; }
 
 const char* collide_variant_name(int v) {
     static const char* names[] = {"bpw1",     "bpw2",         "bpw4",         "bpw1-xcd",     "bpw2-xcd", "bpw4-xcd",
                                   "bpw1-persist", "bpw2-persist", "bpw4-persist", "bpw8",     "bpw8-xcd", "bpw16",
-                                  "bpw16-xcd", "bpw2-xc128", "bpw4-xc64", "bpw4-xc128", "bpw8-xc128"};
-    return v >= 0 && v < 17 ? names[v] : "?";
+                                  "bpw16-xcd", "bpw2-xc128", "bpw4-xc64", "bpw4-xc128", "bpw8-xc128",
+                                  "bpw4-xc256", "bpw8-xc256", "bpw4-xc512", "bpw8-xc512"};
+    return v >= 0 && v < 21 ? names[v] : "?";
 }
 
 template <typename T>
@@ -900,7 +903,11 @@ hipError_t launch_collide(const LevelDev& L, const T* A, T* B, int variant, hipS
         case 13: return collide_variant<T, 2, 4>(L, A, B, s);
         case 14: return collide_variant<T, 4, 3>(L, A, B, s);
         case 15: return collide_variant<T, 4, 4>(L, A, B, s);
-        default: return collide_variant<T, 8, 4>(L, A, B, s);
+        case 16: return collide_variant<T, 8, 4>(L, A, B, s);
+        case 17: return collide_variant<T, 4, 5>(L, A, B, s);
+        case 18: return collide_variant<T, 8, 5>(L, A, B, s);
+        case 19: return collide_variant<T, 4, 6>(L, A, B, s);
+        default: return collide_variant<T, 8, 6>(L, A, B, s);
     }
 }
 

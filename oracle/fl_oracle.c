@@ -226,19 +226,25 @@ void or_hash_query(const uint64_t* tkeys, const uint32_t* tvals, uint64_t cap,
 /* ------------------------------------------------------------------------ */
 /* Cell lookup shared by neighbour, merge and balance restatements: a map
  * from the unique cell key (level, coords) to the smallest index holding it. */
-typedef struct { uint64_t* k; uint32_t* v; uint64_t cap; } cellmap;
+typedef struct { uint64_t* k; uint32_t* v; uint64_t cap; int bits; } cellmap;
+
+/* the slot is the top bits of the Fibonacci product: the packed key carries z
+ * in its low bits, so low product bits would cluster cells that share y, z */
+static uint64_t cm_slot(const cellmap* m, uint64_t k) { return (k * 0x9e3779b97f4a7c15ull) >> (64 - m->bits); }
 
 static void cm_build(cellmap* m, const or_cells* c, const uint64_t* keys) {
     uint64_t cap = 1024;
-    while (cap < 2 * (uint64_t)c->n + 16) cap <<= 1;
+    int bits = 10;
+    while (cap < 2 * (uint64_t)c->n + 16) { cap <<= 1; ++bits; }
     m->cap = cap;
+    m->bits = bits;
     m->k = (uint64_t*)malloc(cap * sizeof(uint64_t));
     m->v = (uint32_t*)malloc(cap * sizeof(uint32_t));
     for (uint64_t s = 0; s < cap; ++s) { m->k[s] = OR_EMPTY; m->v[s] = OR_INVALID; }
     for (size_t i = 0; i < c->n; ++i) {
         uint64_t k = keys[i];
         if (k == OR_EMPTY) continue;
-        uint64_t s = (k * 0x9e3779b97f4a7c15ull) & (cap - 1);
+        uint64_t s = cm_slot(m, k);
         for (;;) {
             if (m->k[s] == OR_EMPTY) { m->k[s] = k; m->v[s] = (uint32_t)i; break; }
             if (m->k[s] == k) break; /* keep smallest index */
@@ -247,7 +253,7 @@ static void cm_build(cellmap* m, const or_cells* c, const uint64_t* keys) {
     }
 }
 static uint32_t cm_get(const cellmap* m, uint64_t k) {
-    uint64_t s = (k * 0x9e3779b97f4a7c15ull) & (m->cap - 1);
+    uint64_t s = cm_slot(m, k);
     for (;;) {
         if (m->k[s] == k) return m->v[s];
         if (m->k[s] == OR_EMPTY) return OR_INVALID;

@@ -250,6 +250,8 @@ class CellSet:
               ("state", np.uint8), ("flag", np.int32), ("material", np.uint32))
 
     def __init__(self, n=0, **arrays):
+        if not n and arrays.get("x") is not None:
+            n = len(arrays["x"])  # missing fields get zeros of the given arrays' length
         for name, dt in self.FIELDS:
             a = arrays.get(name)
             setattr(self, name, np.zeros(n, dt) if a is None else np.ascontiguousarray(a, dt))
