@@ -275,8 +275,9 @@ __device__ inline void bgk_dev(R* h, R inv_tau) {
 
 // Per-cell link codes (2 bits per direction in a u64; bit 63 = active fluid
 // leaf): 0 regular pull, 1 bounce-back from a solid, 2 domain wall (moving
-// wall term of the face), 3 outflow copy from the position clamped into the
-// domain.
+// wall term of the face), 3 pressure outlet: the population of the position
+// clamped into the domain, its density set to the instance's rho0
+// (outflow_link).
 constexpr uint64_t kActiveBit = 1ull << 63;
 
 // c2f capture indices per ghost: the covering coarse cell, then its coarse
@@ -317,6 +318,7 @@ struct LevelDev {
     size_t S;                // slot stride = nbricks * 64
     int32_t nactive;         // active bricks (they come first)
     int32_t level;
+    int32_t outlet;          // 1 when a fluid leaf of the level has a pressure-outlet link
 };
 
 __host__ __device__ inline double tau_at(double tau0, int level) {
@@ -342,9 +344,39 @@ __device__ inline int64_t slot_of(const LevelDev& L, int b, int px, int py, int 
     return (int64_t)nb * 64 + (px & 3) + 4 * (py & 3) + 16 * (pz & 3);
 }
 
+// Pressure outlet (non-equilibrium extrapolation at a fixed density): the
+// population of the source cell s2 -- the pulled position clamped into the
+// domain -- with the source's density replaced by rho_out at the source's
+// velocity,
+//   f_i = f_i(s2) + (rho_out - rho_s) w_i (1 + 3 c_i.u_s + 4.5 (c_i.u_s)^2 - 1.5 u_s^2),
+// i.e. feq_i(rho_out, u_s) + f_i(s2) - feq_i(rho_s, u_s) (feq is linear in
+// rho).  A plain copy (zero gradient) leaves the density level free, and
+// under a velocity inlet the mass then grows without bound; this anchors the
+// outlet pressure.  kDev: A holds deviations h = f - w (rho_s = 1 + sum h,
+// j_s = sum c h) and the result is a deviation.
+template <typename T, typename R, bool kDev>
+__device__ inline R outflow_link(const LevelDev& L, const T* __restrict__ A, size_t s2, int i, R rho_out) {
+    R f[Q];
+#pragma unroll
+    for (int j = 0; j < Q; ++j)
+        f[j] = kDev ? Store<T>::dev(A, pidx(j, L.S, s2), j) : Store<T>::load(A, pidx(j, L.S, s2), j);
+    R rho, jx, jy, jz;
+    density_momentum(f, rho, jx, jy, jz);
+    if (kDev) rho += (R)1;
+    const R ir = (R)1 / rho;
+    const R ux = jx * ir, uy = jy * ir, uz = jz * ir;
+    const R cu = (R)cx(i) * ux + (R)cy(i) * uy + (R)cz(i) * uz;
+    const R usq = ux * ux + uy * uy + uz * uz;
+    return f[i] + (rho_out - rho) * (R)wq(i) * ((R)1 + (R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * usq);
+}
+
 // A non-regular link of direction i of `slot` (brick b, local offsets px..):
-// bounce-back from a solid, moving-wall bounce-back, or outflow copy.
-template <typename T, typename R>
+// bounce-back from a solid, moving-wall bounce-back, or the pressure outlet.
+// kOutlet = false leaves the outlet out of the code (levels whose LevelDev
+// says outlet == 0 have no such link): its 19 extra loads, inlined into
+// every direction of the collide's unrolled pull loop, slowed the whole
+// kernel by a quarter even where no link takes them.
+template <typename T, typename R, bool kOutlet = true>
 __device__ inline R special_link(const LevelDev& L, const T* __restrict__ A, int b, size_t slot, int i, unsigned code,
                                  int px, int py, int pz) {
     if (code == kLinkSolid) return Store<T>::load(A, pidx(opp(i), L.S, slot), opp(i));
@@ -362,7 +394,8 @@ __device__ inline R special_link(const LevelDev& L, const T* __restrict__ A, int
         }
         return Store<T>::load(A, pidx(opp(i), L.S, slot), opp(i)) + (R)6 * (R)wq(i) * (R)I.rho0 * uw;
     }
-    // outflow: the source clamped into the domain along the axes it leaves
+    if constexpr (!kOutlet) return (R)0;  // unreachable on a level without outlet links
+    // outlet: the source clamped into the domain along the axes it leaves
     // through a non-periodic face; along a periodic axis the offset stays
     // as it is and the brick neighbour table (which wraps periodic faces)
     // resolves the image, as the layout's source_of does
@@ -377,12 +410,12 @@ __device__ inline R special_link(const LevelDev& L, const T* __restrict__ A, int
         else if (g >= e && I.bc_type[2 * a + 1] != FL_BC_PERIODIC) r[a] = e - 1 - c3[a] * 4;
     }
     const int64_t s2 = slot_of(L, b, r[0], r[1], r[2]);
-    return Store<T>::load(A, pidx(i, L.S, (size_t)s2), i);
+    return outflow_link<T, R, false>(L, A, (size_t)s2, i, (R)I.rho0);
 }
 
 // special_link on deviations: w_opp(i) = w_i, so bounce-back and the wall
 // term carry over unchanged.
-template <typename T, typename R>
+template <typename T, typename R, bool kOutlet = true>
 __device__ inline R special_link_dev(const LevelDev& L, const T* __restrict__ A, int b, size_t slot, int i,
                                      unsigned code, int px, int py, int pz) {
     if (code == kLinkSolid) return Store<T>::dev(A, pidx(opp(i), L.S, slot), opp(i));
@@ -400,6 +433,7 @@ __device__ inline R special_link_dev(const LevelDev& L, const T* __restrict__ A,
         }
         return Store<T>::dev(A, pidx(opp(i), L.S, slot), opp(i)) + (R)6 * (R)wq(i) * (R)I.rho0 * uw;
     }
+    if constexpr (!kOutlet) return (R)0;  // unreachable on a level without outlet links
     const int p[3] = {px, py, pz};
     int r[3];
 #pragma unroll
@@ -411,7 +445,7 @@ __device__ inline R special_link_dev(const LevelDev& L, const T* __restrict__ A,
         else if (g >= e && I.bc_type[2 * a + 1] != FL_BC_PERIODIC) r[a] = e - 1 - c3[a] * 4;
     }
     const int64_t s2 = slot_of(L, b, r[0], r[1], r[2]);
-    return Store<T>::dev(A, pidx(i, L.S, (size_t)s2), i);
+    return outflow_link<T, R, true>(L, A, (size_t)s2, i, (R)I.rho0);
 }
 
 // Offset of the source that cell (lx, ly, lz) of a brick pulls in direction

@@ -89,7 +89,7 @@ __device__ inline R inv_tau_of(const LevelDev& L, int inst) {
 // position leaves the brick along each axis.  Offsets from a plane's base
 // are 32-bit (below 8 * slots, and the layout keeps slots below 2^27), on a
 // wave-uniform plane base.
-template <typename T>
+template <typename T, bool kOutlet>
 __device__ __attribute__((always_inline)) inline void collide_brick(const LevelDev& L, const T* __restrict__ A,
                                                                     T* __restrict__ B, int b_in) {
     using R = typename Store<T>::R;
@@ -135,9 +135,10 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
                     f[i] = kDevLoads ? Store<T>::dev(A + plane_base(i, S), source(i), i)
                                      : Store<T>::load(A + plane_base(i, S), source(i), i);
                 else
-                    f[i] = kDevLoads ? special_link_dev<T, R>(L, A, b, slot, i, code, lx - cx(i), ly - cy(i),
-                                                              lz - cz(i))
-                                     : special_link<T, R>(L, A, b, slot, i, code, lx - cx(i), ly - cy(i), lz - cz(i));
+                    f[i] = kDevLoads ? special_link_dev<T, R, kOutlet>(L, A, b, slot, i, code, lx - cx(i), ly - cy(i),
+                                                                       lz - cz(i))
+                                     : special_link<T, R, kOutlet>(L, A, b, slot, i, code, lx - cx(i), ly - cy(i),
+                                                                   lz - cz(i));
             }
         }
     }
@@ -184,7 +185,8 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
 // On the 3-level sphere's finest level (980k bricks) the chunked forms are
 // the fastest: mode 1 keeps neighbours in one L2 but streams eight distant
 // windows, mode 0 streams one window but splits neighbours over all L2s.
-template <typename T, int BPW, int MODE>
+// kOutlet: the level has pressure-outlet links (LevelDev::outlet).
+template <typename T, int BPW, int MODE, bool kOutlet>
 __global__ __launch_bounds__(64 * BPW) void k_collide(LevelDev L, const T* __restrict__ A, T* __restrict__ B,
                                                       unsigned ngroups) {
     const int wave = (int)(threadIdx.x >> 6);
@@ -193,14 +195,27 @@ __global__ __launch_bounds__(64 * BPW) void k_collide(LevelDev L, const T* __res
         const unsigned per = (ngroups + gridDim.x - 1) / gridDim.x;
         const unsigned g0 = wg * per;
         const unsigned g1 = g0 + per < ngroups ? g0 + per : ngroups;
-        for (unsigned g = g0; g < g1; ++g) collide_brick<T>(L, A, B, (int)(g * BPW) + wave);
+        for (unsigned g = g0; g < g1; ++g) collide_brick<T, kOutlet>(L, A, B, (int)(g * BPW) + wave);
     } else {
         constexpr unsigned kChunk = MODE == 3 ? 64u : MODE == 4 ? 128u : MODE == 5 ? 256u : 512u;
         const unsigned g = MODE == 1   ? xcd_remap(blockIdx.x, gridDim.x)
                            : MODE >= 3 ? xcd_chunk_remap(blockIdx.x, gridDim.x, kChunk / BPW)
                                        : blockIdx.x;
-        collide_brick<T>(L, A, B, (int)(g * BPW) + wave);
+        collide_brick<T, kOutlet>(L, A, B, (int)(g * BPW) + wave);
     }
+}
+
+// *flag |= 1 when some active slot of the level has a pressure-outlet link
+// (a 2-bit link code of 3 among its 19 directions).
+__global__ __launch_bounds__(kWG) void k_any_outlet(const uint64_t* __restrict__ cflags, size_t n,
+                                                    uint32_t* __restrict__ flag) {
+    constexpr uint64_t kEven = 0x1555555555ull;  // bit 2i of each direction i = 0..18
+    bool any = false;
+    for (size_t s = (size_t)blockIdx.x * kWG + threadIdx.x; s < n; s += (size_t)gridDim.x * kWG) {
+        const uint64_t w = cflags[s];
+        if ((w & kActiveBit) && (w & (w >> 1) & kEven)) any = true;
+    }
+    if (__ballot(any) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
 }
 
 __global__ void k_tick(uint32_t* tick) {
@@ -850,7 +865,7 @@ int device_cus() {
     return cus;
 }
 
-template <typename T, int BPW, int MODE>
+template <typename T, int BPW, int MODE, bool kOutlet = false>
 hipError_t collide_variant(const LevelDev& L, const T* A, T* B, hipStream_t s) {
     const unsigned ngroups = (unsigned)((L.nactive + BPW - 1) / BPW);
     unsigned grid = ngroups;
@@ -858,7 +873,7 @@ hipError_t collide_variant(const LevelDev& L, const T* A, T* B, hipStream_t s) {
         static int per_cu = 0;
         if (per_cu == 0) {
             int nb = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<T, BPW, MODE>, 64 * BPW, 0) !=
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_collide<T, BPW, MODE, kOutlet>, 64 * BPW, 0) !=
                     hipSuccess ||
                 nb <= 0)
                 nb = 8 / BPW > 0 ? 8 / BPW : 1;
@@ -867,7 +882,7 @@ hipError_t collide_variant(const LevelDev& L, const T* A, T* B, hipStream_t s) {
         const unsigned cap = (unsigned)(per_cu * device_cus());
         if (grid > cap) grid = cap;
     }
-    hipLaunchKernelGGL((k_collide<T, BPW, MODE>), dim3(grid), dim3(64 * BPW), 0, s, L, A, B, ngroups);
+    hipLaunchKernelGGL((k_collide<T, BPW, MODE, kOutlet>), dim3(grid), dim3(64 * BPW), 0, s, L, A, B, ngroups);
     return hipGetLastError();
 }
 
@@ -886,6 +901,16 @@ const char* collide_variant_name(int v) {
 template <typename T>
 hipError_t launch_collide(const LevelDev& L, const T* A, T* B, int variant, hipStream_t s) {
     if (L.nactive == 0) return hipSuccess;
+    if (L.outlet) {
+        // levels with pressure-outlet links (at most the few that touch an
+        // outflow face) run one of three outlet instances, by the variant's
+        // workgroup-to-brick mapping; every variant gives the same bits
+        switch (variant) {
+            case 3: case 4: case 5: case 10: case 12: return collide_variant<T, 4, 1, true>(L, A, B, s);
+            case 0: case 1: case 2: case 6: case 7: case 8: case 9: case 11: return collide_variant<T, 4, 0, true>(L, A, B, s);
+            default: return collide_variant<T, 4, 4, true>(L, A, B, s);
+        }
+    }
     switch (variant) {
         case 0: return collide_variant<T, 1, 0>(L, A, B, s);
         case 1: return collide_variant<T, 2, 0>(L, A, B, s);
@@ -909,6 +934,11 @@ hipError_t launch_collide(const LevelDev& L, const T* A, T* B, int variant, hipS
         case 19: return collide_variant<T, 4, 6>(L, A, B, s);
         default: return collide_variant<T, 8, 6>(L, A, B, s);
     }
+}
+
+hipError_t launch_any_outlet(const uint64_t* cflags, size_t nslots, uint32_t* flag, hipStream_t s) {
+    if (nslots) hipLaunchKernelGGL(k_any_outlet, dim3(lgrid(nslots)), dim3(kWG), 0, s, cflags, nslots, flag);
+    return hipGetLastError();
 }
 
 hipError_t launch_tick(uint32_t* tick, hipStream_t s) {

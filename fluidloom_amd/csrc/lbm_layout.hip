@@ -455,7 +455,8 @@ __global__ __launch_bounds__(kB) void k_links(size_t nb, int l, const int32_t* _
                     const uint8_t t2 = s2 < 0 ? T_NONE : stype[s2];
                     if (regular_t(t2)) {
                         code = kLinkOutflow;
-                        if (t2 != T_FLUID) atomicOr(&needed[s2], 1u << i);
+                        // the outlet reads every population of its source
+                        if (t2 != T_FLUID) atomicOr(&needed[s2], (1u << Q) - 1u);
                     } else if (t2 == T_SOLID || t2 == T_WALL) {
                         code = kLinkSolid;
                     } else {

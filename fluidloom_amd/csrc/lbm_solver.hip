@@ -598,7 +598,9 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
                             uint8_t t2 = s2 < 0 ? T_NONE : H.stype[(size_t)s2];
                             if (regular(t2)) {
                                 code = kLinkOutflow;
-                                if (t2 != T_FLUID) __atomic_fetch_or(&H.needed[(size_t)s2], 1u << i, __ATOMIC_RELAXED);
+                                // the outlet reads every population of its source
+                                if (t2 != T_FLUID)
+                                    __atomic_fetch_or(&H.needed[(size_t)s2], (1u << Q) - 1u, __ATOMIC_RELAXED);
                             } else if (t2 == T_SOLID || t2 == T_WALL) {
                                 code = kLinkSolid;
                             } else {
@@ -773,7 +775,7 @@ fl_status build_layout(fl_lbm_solver* S, const int32_t* X, const int32_t* Y, con
 uint64_t layout_bytes(const fl_lbm_solver* S) {
     const size_t sb = store_bytes(S->storage), cb = compute_bytes(S->storage);
     const int NI = S->ninst();
-    uint64_t t = (uint64_t)NI * (sizeof(InstDev) + 4 * sizeof(double));
+    uint64_t t = (uint64_t)NI * (sizeof(InstDev) + 4 * sizeof(double)) + (uint64_t)S->L * sizeof(uint32_t);
     for (int l = 0; l < S->L; ++l) {
         const HostLevel& H = S->h[l];
         t += H.nbr.size() * 4 + H.bcoord.size() * 4 + H.binst.size() * 2 + H.bflags.size() * 4 + H.cflags.size() * 8;
@@ -870,7 +872,17 @@ fl_status upload_and_init(fl_lbm_solver* S, bool tables_on_device) {
         D.ld.tick = S->ticks + l;
         FL_HIP(launch_init<T>(D.ld, static_cast<T*>(D.buf[0]), static_cast<T*>(D.buf[1]), S->stream));
     }
+    // which levels have pressure-outlet links: the collide runs its outlet
+    // instances only there (the link words decide, whichever builder made them)
+    uint32_t* flags = nullptr;
+    if ((r = dev_alloc(S, reinterpret_cast<void**>(&flags), (size_t)S->L * sizeof(uint32_t)))) return r;
+    FL_HIP(hipMemsetAsync(flags, 0, (size_t)S->L * sizeof(uint32_t), S->stream));
+    for (int l = 0; l < S->L; ++l)
+        FL_HIP(launch_any_outlet(S->d[l].cflags, (size_t)S->h[l].nactive * 64, flags + l, S->stream));
+    std::vector<uint32_t> hf((size_t)S->L);
+    FL_HIP(hipMemcpyAsync(hf.data(), flags, (size_t)S->L * sizeof(uint32_t), hipMemcpyDeviceToHost, S->stream));
     FL_HIP(hipStreamSynchronize(S->stream));
+    for (int l = 0; l < S->L; ++l) S->d[l].ld.outlet = hf[(size_t)l] ? 1 : 0;
     return FL_OK;
 }
 

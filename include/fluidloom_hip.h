@@ -167,8 +167,9 @@ typedef struct {
  * finest cell at level <= L covering the same-level face neighbour, 0xFFFFFFFE
  * when that position is covered only by finer cells, 0xFFFFFFFF outside the
  * domain (level-0 extents root[3], level L extents root << L).  Uses a cell
- * hash (capacity >= fl_hash_capacity(n), power of two) and n u64 of key
- * scratch that the caller provides; the table is rebuilt here. */
+ * hash (capacity >= fl_hash_capacity(n), power of two; d_tkeys 16 B aligned,
+ * as any hipMalloc pointer is) and n u64 of key scratch that the caller
+ * provides; the table is rebuilt here. */
 fl_status fl_face_neighbors(const fl_cells* c, size_t n, const int32_t* root, uint64_t* d_keys_scratch,
                             uint64_t* d_tkeys, uint32_t* d_tvals, uint64_t capacity, uint32_t* d_nbr,
                             fl_stream s);
@@ -204,6 +205,9 @@ typedef struct {
     int32_t pad;
 } fl_regrid_result;
 
+/* Workspace of fl_regrid / fl_balance / fl_merge / fl_gen_flags for n cells
+ * (16 B aligned, as any hipMalloc pointer is): keys, the cell map at twice
+ * fl_hash_capacity(n) slots, shadow levels, group ids and scan words. */
 size_t fl_regrid_temp_bytes(size_t n);
 /* in/out cells may not alias; out must hold n + 7*n cells (worst case).
  * d_fields_in/out: AoS [cell][ncomp] float (may be NULL when ncomp == 0). */

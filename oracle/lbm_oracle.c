@@ -10,7 +10,10 @@
  *  - Stored state: post-collision populations f*; pull streaming
  *    f_i(x) = f*_i(x - c_i); BGK f* = f - (f - feq)/tau.
  *  - Walls: half-way bounce-back f_i(x) = f*_opp(x) + 6 w_i rho0 (c_i.u_w)
- *    (Ladd); outflow faces copy from the position clamped into the domain.
+ *    (Ladd).  Outflow faces are pressure outlets: the population of the
+ *    position clamped into the domain, with that cell's density replaced by
+ *    rho0 at its velocity, f_i = f*_i(s) + (rho0 - rho_s) w_i (1 + 3 c.u_s +
+ *    4.5 (c.u_s)^2 - 1.5 u_s^2) (non-equilibrium extrapolation).
  *  - Level l: dx = 2^-l, dt = 2^-l, tau_l = 2^l (tau0 - 1/2) + 1/2.
  *  - Coarse step advance(l): f2c fill, capture pre-collision state of c2f
  *    sources (from the previous buffer), collide-stream, capture again, then
@@ -177,8 +180,19 @@ static void gather(const or_lbm* s, const level_t* v, int64_t pos, const real* b
             continue;
         }
         int64_t sp = pidx(v, sx, sy, sz);
-        if (regular(v->type[sp])) out[i] = buf[(size_t)i * N + sp];
-        else out[i] = buf[(size_t)OPP[i] * N + pos];
+        if (!regular(v->type[sp])) {
+            out[i] = buf[(size_t)OPP[i] * N + pos];
+        } else if (!o) {
+            out[i] = buf[(size_t)i * N + sp];
+        } else {
+            /* pressure outlet: the clamped source's population with its
+             * density set to rho0 at its own velocity (non-equilibrium
+             * extrapolation; feq is linear in rho) */
+            real fs[Q], rs, us, vs, ws;
+            for (int j = 0; j < Q; ++j) fs[j] = buf[(size_t)j * N + sp];
+            moments(fs, &rs, &us, &vs, &ws);
+            out[i] = fs[i] + (s->p.rho0 - rs) * feq(1.0, us, vs, ws, i);
+        }
     }
 }
 

@@ -168,6 +168,32 @@ def test_face_neighbors_match_oracle(fl, oracle):
     assert np.array_equal(amr.face_neighbors(g, (1, 1, 1)).to_host()[: 6 * c.n], oracle.face_neighbors(c, (1, 1, 1)))
 
 
+def test_face_neighbors_dense_blocks_at_high_load(fl, oracle):
+    """The cell map's worst case: every cell in a full 2x2x2 block (a
+    uniform 84^3 region at level 7, a refined 32^3 patch inside it, and a
+    shuffled copy of the same cells), so whole blocks collide as units, at
+    the public capacity's load (the uniform region: 593K cells in 2^20
+    slots, 0.57; with the patch: 822K cells in 2^21 slots, 0.39).  Face neighbours must equal the
+    oracle's, whose own map probes per cell."""
+    from fluidloom_amd import amr
+    g84 = np.stack(np.meshgrid(*(np.arange(84),) * 3, indexing="ij"), -1).reshape(-1, 3).astype(np.int32)
+    c = oracle.CellSet(len(g84), x=g84[:, 0], y=g84[:, 1], z=g84[:, 2], level=np.full(len(g84), 7, np.uint8))
+    assert 0.55 < c.n / float(fl.lib().fl_hash_capacity(c.n)) < 0.6
+    assert np.array_equal(amr.face_neighbors(_gpu_cells(c), (1, 1, 1)).to_host()[: 6 * c.n],
+                          oracle.face_neighbors(c, (1, 1, 1)))
+    # replace the level-7 cells of [16, 48)^3 by their level-8 children
+    inside = np.all((g84 >= 16) & (g84 < 48), axis=1)
+    kids = (2 * g84[inside][:, None, :] + np.array([[(q >> a) & 1 for a in range(3)] for q in range(8)],
+                                                   np.int32)[None]).reshape(-1, 3)
+    xyz = np.concatenate([g84[~inside], kids])
+    lv = np.concatenate([np.full((~inside).sum(), 7, np.uint8), np.full(len(kids), 8, np.uint8)])
+    perm = np.random.default_rng(3).permutation(len(xyz))
+    for order in (np.arange(len(xyz)), perm):
+        c = oracle.CellSet(len(xyz), x=xyz[order, 0], y=xyz[order, 1], z=xyz[order, 2], level=lv[order])
+        assert np.array_equal(amr.face_neighbors(_gpu_cells(c), (1, 1, 1)).to_host()[: 6 * c.n],
+                              oracle.face_neighbors(c, (1, 1, 1)))
+
+
 def _same_cells(gh, oc):
     for nm in ("x", "y", "z", "level", "state", "material"):
         assert np.array_equal(gh[nm], getattr(oc, nm)), nm
