@@ -208,6 +208,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default collide variant")
     ap.add_argument("--variant", default=None, help="force this collide variant on every level (no autotune)")
     args = ap.parse_args()
+    if args.steps < 1:
+        print("bench.py: --steps must be at least 1", file=sys.stderr)
+        sys.exit(2)
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
         sys.exit(2)

@@ -16,9 +16,11 @@ void set_error(const std::string& msg);
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
 
 // Device blocks from the library's block cache (dev_cache.hip) on the
-// current device; cache_free keeps the block for later requests once stream
-// s (the stream of the block's last user) has drained.
-fl_status cache_alloc(void** p, size_t bytes);
+// current device.  cache_free keeps the block with s, the stream of its last
+// user; cache_alloc for stream s hands it out at once when it was last used
+// on s, else after that stream has passed the free (a null s waits on the
+// host).
+fl_status cache_alloc(void** p, size_t bytes, hipStream_t s);
 void cache_free(void* p, hipStream_t s);
 
 constexpr uint64_t kEmptyKey = 0xFFFFFFFFFFFFFFFFull;

@@ -470,10 +470,14 @@ __global__ __launch_bounds__(kWG) void k_fill_f2c(LevelDev Lc, const uint32_t* _
                            rb);
 }
 
+// skip_simple: a regrid's population transfer writes every fluid leaf of the
+// new layout in both buffers next, so the bricks whose 64 cells are all
+// fluid leaves ("simple") need no initial state.
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_init(LevelDev L, T* __restrict__ A, T* __restrict__ B) {
+__global__ __launch_bounds__(kWG) void k_init(LevelDev L, T* __restrict__ A, T* __restrict__ B, int skip_simple) {
     using R = typename Store<T>::R;
     for (size_t s = (size_t)blockIdx.x * kWG + threadIdx.x; s < L.S; s += (size_t)gridDim.x * kWG) {
+        if (skip_simple && (s >> 6) < (size_t)L.nactive && (L.bflags[s >> 6] & 1u)) continue;
         const InstDev& I = L.inst[L.binst[s >> 6]];
         const R ux = (R)I.u_init[0], uy = (R)I.u_init[1], uz = (R)I.u_init[2], rho = (R)I.rho_init;
         const R usq = (R)(I.u_init[0] * I.u_init[0] + I.u_init[1] * I.u_init[1] + I.u_init[2] * I.u_init[2]);
@@ -720,16 +724,27 @@ __global__ __launch_bounds__(kWG) void k_vorticity(LevelDev L, const T* __restri
 }
 
 // Population transfer of a regrid: copy / split child / merged parent / rest.
+// The copy moves the stored bits (an unsigned integer of T's size); the 19
+// loads of a cell are issued before its 38 non-temporal stores.
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_remap_copy(const T* __restrict__ Ao, size_t So, const uint32_t* __restrict__ dst,
-                                                    const uint32_t* __restrict__ src, uint32_t n, T* __restrict__ A,
-                                                    T* __restrict__ B, size_t S) {
+__global__ __launch_bounds__(kWG) void k_remap_copy(const T* __restrict__ Ao_, size_t So, const uint32_t* __restrict__ dst,
+                                                    const uint32_t* __restrict__ src, uint32_t n, T* __restrict__ A_,
+                                                    T* __restrict__ B_, size_t S) {
+    using U = typename std::conditional<sizeof(T) == 2, uint16_t,
+                                        typename std::conditional<sizeof(T) == 4, uint32_t, uint64_t>::type>::type;
+    static_assert(sizeof(U) == sizeof(T), "storage element size");
+    const U* __restrict__ Ao = reinterpret_cast<const U*>(Ao_);
+    U* __restrict__ A = reinterpret_cast<U*>(A_);
+    U* __restrict__ B = reinterpret_cast<U*>(B_);
     for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < n; k += gridDim.x * kWG) {
+        const uint32_t s0 = src[k], d0 = dst[k];
+        U v[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) v[i] = Ao[pidx(i, So, s0)];
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            const T v = Ao[pidx(i, So, src[k])];
-            A[pidx(i, S, dst[k])] = v;
-            B[pidx(i, S, dst[k])] = v;
+            __builtin_nontemporal_store(v[i], A + pidx(i, S, d0));
+            __builtin_nontemporal_store(v[i], B + pidx(i, S, d0));
         }
     }
 }
@@ -978,9 +993,9 @@ hipError_t launch_fill_f2c(const LevelDev& Lc, const uint32_t* ghost, const uint
 }
 
 template <typename T>
-hipError_t launch_init(const LevelDev& L, T* A, T* B, hipStream_t s) {
+hipError_t launch_init(const LevelDev& L, T* A, T* B, hipStream_t s, bool skip_simple) {
     if (L.S == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_init<T>, dim3(lgrid(L.S)), dim3(kWG), 0, s, L, A, B);
+    hipLaunchKernelGGL(k_init<T>, dim3(lgrid(L.S)), dim3(kWG), 0, s, L, A, B, skip_simple ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -1091,7 +1106,7 @@ hipError_t launch_ghost_eq(const uint32_t* c2f_gchild, const uint32_t* c2f_gcap,
                                            uint32_t, const void*, const void*, uint32_t, double, T*, hipStream_t); \
     template hipError_t launch_fill_f2c<T>(const LevelDev&, const uint32_t*, const uint32_t*, uint32_t,         \
                                            const LevelDev&, const T*, T*, hipStream_t);                         \
-    template hipError_t launch_init<T>(const LevelDev&, T*, T*, hipStream_t);                                   \
+    template hipError_t launch_init<T>(const LevelDev&, T*, T*, hipStream_t, bool);                             \
     template hipError_t launch_brick_sums<T>(const LevelDev&, const T*, double, double*, hipStream_t);          \
     template hipError_t launch_brick_du<T>(const LevelDev&, const T*, float*, float*, float, double*, hipStream_t); \
     template hipError_t launch_velocity_sample<T>(const LevelDev&, const T*, float*, hipStream_t);               \

@@ -31,7 +31,7 @@ template <typename T>
 hipError_t launch_fill_f2c(const LevelDev& Lc, const uint32_t* ghost, const uint32_t* child8, uint32_t n,
                            const LevelDev& Lf, const T* Af, T* Ac, hipStream_t s);
 template <typename T>
-hipError_t launch_init(const LevelDev& L, T* A, T* B, hipStream_t s);
+hipError_t launch_init(const LevelDev& L, T* A, T* B, hipStream_t s, bool skip_simple);
 // FP64 reductions: per-brick (or per-link) values, then one workgroup per
 // instance over its contiguous range (fixed order: results do not depend on
 // the other instances of the batch).

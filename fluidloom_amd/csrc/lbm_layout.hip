@@ -767,9 +767,8 @@ __global__ void k_merge_src(LevelMaps om, const LayoutInst* __restrict__ inst, i
 }
 
 // ------------------------------------------------------- host helpers
-// Temporary device memory of one build, freed when it goes out of scope.
-// Temporaries of one scope, from the block cache; returned to it (after
-// stream s drains) when the scope ends.
+// Temporaries of one scope on stream s, from the block cache; returned to it
+// when the scope ends.
 struct Tmp {
     hipStream_t s;
     std::vector<void*> p;
@@ -782,7 +781,7 @@ struct Tmp {
     template <typename T>
     fl_status get(T** out, size_t n) {
         void* q = nullptr;
-        const fl_status st = cache_alloc(&q, (n ? n : 1) * sizeof(T));
+        const fl_status st = cache_alloc(&q, (n ? n : 1) * sizeof(T), s);
         if (st) return st;
         p.push_back(q);
         *out = static_cast<T*>(q);

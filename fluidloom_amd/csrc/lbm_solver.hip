@@ -269,7 +269,12 @@ struct fl_lbm_solver {
         if (qslots) (void)hipFree(qslots);
         if (qbuf) (void)hipFree(qbuf);
         if (own_ticks && ticks) (void)hipFree(ticks);
-        if (own_stream && stream) (void)hipStreamDestroy(stream);
+        // the blocks cached above are tagged with this stream: its work ends
+        // before the handle can be reused by a new stream
+        if (own_stream && stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
     }
     int ninst() const { return (int)inst.size(); }
 };
@@ -282,7 +287,7 @@ template <typename T>
 fl_status dev_upload(fl_lbm_solver* S, T** dst, const std::vector<T>& v) {
     *dst = nullptr;
     if (v.empty()) return FL_OK;
-    FL_TRY(fl::cache_alloc(reinterpret_cast<void**>(dst), v.size() * sizeof(T)));
+    FL_TRY(fl::cache_alloc(reinterpret_cast<void**>(dst), v.size() * sizeof(T), S->stream));
     S->allocs.push_back(*dst);
     S->device_bytes += v.size() * sizeof(T);
     FL_HIP(hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, S->stream));
@@ -292,7 +297,7 @@ fl_status dev_upload(fl_lbm_solver* S, T** dst, const std::vector<T>& v) {
 fl_status dev_alloc(fl_lbm_solver* S, void** dst, size_t bytes) {
     *dst = nullptr;
     if (bytes == 0) return FL_OK;
-    FL_TRY(fl::cache_alloc(dst, bytes));
+    FL_TRY(fl::cache_alloc(dst, bytes, S->stream));
     S->allocs.push_back(*dst);
     S->device_bytes += bytes;
     return FL_OK;
@@ -822,9 +827,11 @@ fl_status upload_host_tables(fl_lbm_solver* S, int l) {
 }
 
 // Buffers, per-level constants and the initial state; the layout tables are
-// uploaded first unless the device builder made them.
+// uploaded first unless the device builder made them.  remap_follows: a
+// regrid's population transfer writes every fluid leaf next (bricks of fluid
+// leaves only are then not initialised).
 template <typename T>
-fl_status upload_and_init(fl_lbm_solver* S, bool tables_on_device) {
+fl_status upload_and_init(fl_lbm_solver* S, bool tables_on_device, bool remap_follows) {
     const size_t sb = sizeof(T);
     const size_t cb = compute_bytes(S->storage);
     const int NI = S->ninst();
@@ -870,7 +877,7 @@ fl_status upload_and_init(fl_lbm_solver* S, bool tables_on_device) {
         D.ld.nactive = (int32_t)H.nactive;
         D.ld.level = l;
         D.ld.tick = S->ticks + l;
-        FL_HIP(launch_init<T>(D.ld, static_cast<T*>(D.buf[0]), static_cast<T*>(D.buf[1]), S->stream));
+        FL_HIP(launch_init<T>(D.ld, static_cast<T*>(D.buf[0]), static_cast<T*>(D.buf[1]), S->stream, remap_follows));
     }
     // which levels have pressure-outlet links: the collide runs its outlet
     // instances only there (the link words decide, whichever builder made them)
@@ -1362,12 +1369,12 @@ fl_status attach(fl_lbm_solver* S, hipStream_t stream, const fl_lbm_solver* prev
     return FL_OK;
 }
 
-fl_status init_any(fl_lbm_solver* S, bool tables_on_device) {
+fl_status init_any(fl_lbm_solver* S, bool tables_on_device, bool remap_follows) {
     switch (S->storage) {
-        case FL_STORE_F32: return upload_and_init<float>(S, tables_on_device);
-        case FL_STORE_F16: return upload_and_init<_Float16>(S, tables_on_device);
-        case FL_STORE_F16C: return upload_and_init<fp16c>(S, tables_on_device);
-        default: return upload_and_init<double>(S, tables_on_device);
+        case FL_STORE_F32: return upload_and_init<float>(S, tables_on_device, remap_follows);
+        case FL_STORE_F16: return upload_and_init<_Float16>(S, tables_on_device, remap_follows);
+        case FL_STORE_F16C: return upload_and_init<fp16c>(S, tables_on_device, remap_follows);
+        default: return upload_and_init<double>(S, tables_on_device, remap_follows);
     }
 }
 
@@ -1382,7 +1389,7 @@ fl_status build_on_device(const fl_lbm_params* params, int ninst, const size_t* 
     FL_REQUIRE(n == 0 || (d_x && d_y && d_z && d_level && d_state), "null leaf arrays");
     if ((r = attach(S.get(), stream, prev))) return r;
     if ((r = device_layout(S.get(), d_x, d_y, d_z, d_level, d_state, n))) return r;
-    if ((r = init_any(S.get(), true))) return r;
+    if ((r = init_any(S.get(), true, prev != nullptr))) return r;
     out = std::move(S);
     return FL_OK;
 }
@@ -1396,7 +1403,7 @@ fl_status build_on_host(const fl_lbm_params* params, int ninst, const size_t* of
     fl_status r = plan_solver(params, ninst, offsets, h_x, h_y, h_z, h_level, h_state, S);
     if (r) return r;
     if ((r = attach(S.get(), stream, prev))) return r;
-    if ((r = init_any(S.get(), false))) return r;
+    if ((r = init_any(S.get(), false, prev != nullptr))) return r;
     out = std::move(S);
     return FL_OK;
 }
@@ -1473,10 +1480,10 @@ template <typename T>
 fl_status remap(fl_lbm_solver* O, fl_lbm_solver* N) {
     std::vector<void*> tmp_allocs;
     fl_status st = FL_OK;
-    const DevAlloc talloc = [&tmp_allocs](void** p, size_t b) -> fl_status {
+    const DevAlloc talloc = [&tmp_allocs, N](void** p, size_t b) -> fl_status {
         *p = nullptr;
         if (b == 0) return FL_OK;
-        FL_TRY(fl::cache_alloc(p, b));
+        FL_TRY(fl::cache_alloc(p, b, N->stream));
         tmp_allocs.push_back(*p);
         return FL_OK;
     };
