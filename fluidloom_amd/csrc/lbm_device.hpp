@@ -308,7 +308,8 @@ struct InstDev {
 struct LevelDev {
     const int32_t* nbr;      // [brick][27], -1 = none
     const int32_t* bcoord;   // [brick][3]
-    const uint32_t* bflags;  // bit 0: simple brick (all 64 active, all links regular)
+    const uint32_t* bflags;  // bit 0: simple brick (all 64 active, all links regular);
+                             // bit 1: a cell of the brick has a pressure-outlet link
     const uint64_t* cflags;  // [slot]
     const uint16_t* binst;   // [brick] instance id
     const float* itau;       // [instance] 1 / tau at this level

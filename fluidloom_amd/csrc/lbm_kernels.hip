@@ -95,6 +95,13 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
     using R = typename Store<T>::R;
     const int b = __builtin_amdgcn_readfirstlane(b_in);
     if (b >= L.nactive) return;
+    if constexpr (kOutlet) {
+        // only the bricks with an outlet link take the outlet code
+        if (!(L.bflags[b] & 2u)) {
+            collide_brick<T, false>(L, A, B, b);
+            return;
+        }
+    }
     const int inst = L.binst[b];
     if (!L.inst[inst].live) return;
     const int lane = (int)(threadIdx.x & 63);
@@ -205,15 +212,19 @@ __global__ __launch_bounds__(64 * BPW) void k_collide(LevelDev L, const T* __res
     }
 }
 
-// *flag |= 1 when some active slot of the level has a pressure-outlet link
-// (a 2-bit link code of 3 among its 19 directions).
-__global__ __launch_bounds__(kWG) void k_any_outlet(const uint64_t* __restrict__ cflags, size_t n,
-                                                    uint32_t* __restrict__ flag) {
+// Bricks with a pressure-outlet link (a 2-bit link code of 3 among a cell's
+// 19 directions) get bit 1 of their flags, and *flag |= 1 when the level has
+// any.  A wave covers one brick per round (the grid stride is a multiple of
+// 64).
+__global__ __launch_bounds__(kWG) void k_any_outlet(const uint64_t* __restrict__ cflags, uint32_t* __restrict__ bflags,
+                                                    size_t n, uint32_t* __restrict__ flag) {
     constexpr uint64_t kEven = 0x1555555555ull;  // bit 2i of each direction i = 0..18
     bool any = false;
     for (size_t s = (size_t)blockIdx.x * kWG + threadIdx.x; s < n; s += (size_t)gridDim.x * kWG) {
         const uint64_t w = cflags[s];
-        if ((w & kActiveBit) && (w & (w >> 1) & kEven)) any = true;
+        const bool o = (w & kActiveBit) && (w & (w >> 1) & kEven);
+        if (__ballot(o) && (threadIdx.x & 63) == 0) atomicOr(&bflags[s >> 6], 2u);
+        any = any || o;
     }
     if (__ballot(any) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
 }
@@ -951,8 +962,8 @@ hipError_t launch_collide(const LevelDev& L, const T* A, T* B, int variant, hipS
     }
 }
 
-hipError_t launch_any_outlet(const uint64_t* cflags, size_t nslots, uint32_t* flag, hipStream_t s) {
-    if (nslots) hipLaunchKernelGGL(k_any_outlet, dim3(lgrid(nslots)), dim3(kWG), 0, s, cflags, nslots, flag);
+hipError_t launch_any_outlet(const uint64_t* cflags, uint32_t* bflags, size_t nslots, uint32_t* flag, hipStream_t s) {
+    if (nslots) hipLaunchKernelGGL(k_any_outlet, dim3(lgrid(nslots)), dim3(kWG), 0, s, cflags, bflags, nslots, flag);
     return hipGetLastError();
 }
 

@@ -17,9 +17,10 @@ const char* collide_variant_name(int v);
 template <typename T>
 hipError_t launch_collide(const LevelDev& L, const T* A, T* B, int variant, hipStream_t s);
 // advances a level's stochastic-rounding clock (one kernel node, graph-safe)
-// *flag |= 1 when one of the first nslots link words holds a pressure-outlet
-// link (the caller zeroes *flag)
-hipError_t launch_any_outlet(const uint64_t* cflags, size_t nslots, uint32_t* flag, hipStream_t s);
+// Bit 1 of bflags on every brick (of the first nslots / 64) with a
+// pressure-outlet link, and *flag |= 1 when there is one (the caller zeroes
+// *flag)
+hipError_t launch_any_outlet(const uint64_t* cflags, uint32_t* bflags, size_t nslots, uint32_t* flag, hipStream_t s);
 hipError_t launch_tick(uint32_t* tick, hipStream_t s);
 template <typename T>
 hipError_t launch_capture(const LevelDev& L, const T* A, const uint32_t* slots, uint32_t ncap, void* cap, hipStream_t s);

@@ -885,7 +885,7 @@ fl_status upload_and_init(fl_lbm_solver* S, bool tables_on_device, bool remap_fo
     if ((r = dev_alloc(S, reinterpret_cast<void**>(&flags), (size_t)S->L * sizeof(uint32_t)))) return r;
     FL_HIP(hipMemsetAsync(flags, 0, (size_t)S->L * sizeof(uint32_t), S->stream));
     for (int l = 0; l < S->L; ++l)
-        FL_HIP(launch_any_outlet(S->d[l].cflags, (size_t)S->h[l].nactive * 64, flags + l, S->stream));
+        FL_HIP(launch_any_outlet(S->d[l].cflags, S->d[l].bflags, (size_t)S->h[l].nactive * 64, flags + l, S->stream));
     std::vector<uint32_t> hf((size_t)S->L);
     FL_HIP(hipMemcpyAsync(hf.data(), flags, (size_t)S->L * sizeof(uint32_t), hipMemcpyDeviceToHost, S->stream));
     FL_HIP(hipStreamSynchronize(S->stream));
