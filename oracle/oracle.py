@@ -30,10 +30,14 @@ How each oracle function is pinned (besides the GPU comparison):
                           workload's definition (ranges, Bernoulli rates,
                           merges only on complete sibling groups)
   lbm                     closed forms: equilibrium moments, conservation,
-                          rest state, Couette profile, Taylor-Green decay,
-                          uniform flow through coarse/fine interfaces.  The
-                          reference has no lattice step, so its parity with
-                          the reference is unpinned by construction.
+                          rest state, Couette profile (also across refined
+                          bands), Taylor-Green decay, uniform flow through
+                          coarse/fine interfaces, the momentum-exchange force
+                          on a Couette wall, the pressure-outlet channel's
+                          uniform steady state, y-shift invariance across a
+                          periodic seam at an outlet.  The reference has no
+                          lattice step, so its parity with the reference is
+                          unpinned by construction.
 """
 from __future__ import annotations
 
