@@ -194,9 +194,41 @@ __device__ inline uint32_t mix32(uint32_t x) {
     return x;
 }
 
+// c_i . u from the nonzero components of c_i only (i a compile-time constant
+// after unrolling).  The same value as cx*ux + cy*uy + cz*uz for finite u:
+// the dropped terms are products with 0, which IEEE arithmetic does not let
+// the compiler remove by itself (0 * inf is NaN), and +-1 products are exact.
+template <typename R>
+__device__ __attribute__((always_inline)) inline R cdot(int i, R ux, R uy, R uz) {
+    R s = 0;
+    bool any = false;
+    if (cx(i)) {
+        s = cx(i) > 0 ? ux : -ux;
+        any = true;
+    }
+    if (cy(i)) {
+        const R t = cy(i) > 0 ? uy : -uy;
+        s = any ? s + t : t;
+        any = true;
+    }
+    if (cz(i)) {
+        const R t = cz(i) > 0 ? uz : -uz;
+        s = any ? s + t : t;
+    }
+    return s;
+}
+
+template <typename R>
+__device__ __attribute__((always_inline)) inline R fmadd(R a, R b, R c) {
+    if constexpr (sizeof(R) == 8)
+        return __builtin_fma(a, b, c);
+    else
+        return __builtin_fmaf(a, b, c);
+}
+
 template <typename R>
 __device__ inline R feq(int i, R rho, R ux, R uy, R uz, R usq) {
-    const R cu = (R)cx(i) * ux + (R)cy(i) * uy + (R)cz(i) * uz;
+    const R cu = cdot(i, ux, uy, uz);
     return (R)wq(i) * rho * ((R)1 + (R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * usq);
 }
 
@@ -241,7 +273,7 @@ __device__ inline void bgk(R* f, R inv_tau) {
 // with d = rho - 1 (from sum_i w_i = 1 and sum_i w_i c_i = 0).
 template <typename R>
 __device__ inline R feq_dev(int i, R d, R rho, R ux, R uy, R uz, R usq) {
-    const R cu = (R)cx(i) * ux + (R)cy(i) * uy + (R)cz(i) * uz;
+    const R cu = cdot(i, ux, uy, uz);
     return (R)wq(i) * (d + rho * ((R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * usq));
 }
 
@@ -262,15 +294,66 @@ __device__ inline void dev_moments(const R* h, R& d, R& rho, R& ux, R& uy, R& uz
     uz = jz * ir;
 }
 
-// BGK on deviations: h* = h - (h - (feq - w)) / tau (in place); the same
-// update as bgk() on f = w + h.
+// BGK on deviations, h* = h - (h - (feq - w)) / tau (in place): the same
+// update as bgk() on f = w + h, arranged over the opposite pairs (i, i+1)
+// of the lattice, which share |c_i . u| and the weight.  With omega = 1 / tau
+// and c0 = d - 1.5 rho u^2,
+//   omega (feq - w)_i = omega w (c0 + 4.5 rho cu^2) +- 3 omega w rho cu = S +- A,
+//   h*_i = (1 - omega) h_i + S + A,  h*_{i+1} = (1 - omega) h_{i+1} + S - A,
+// and density and momentum come from the pair sums and differences.  About
+// 130 operations per cell against 240 for the direction-by-direction form
+// (the collide kernels in 16-bit storage are bound by instruction issue as
+// much as by memory).  1 / rho in FP32 is the hardware reciprocal (1 ulp).
 template <typename R>
-__device__ inline void bgk_dev(R* h, R inv_tau) {
-    R d, rho, ux, uy, uz;
-    dev_moments(h, d, rho, ux, uy, uz);
-    const R usq = ux * ux + uy * uy + uz * uz;
+__device__ __attribute__((always_inline)) inline void bgk_dev(R* h, R inv_tau) {
+    constexpr int P = (Q - 1) / 2;
+    R sum[P], dif[P];
 #pragma unroll
-    for (int i = 0; i < Q; ++i) h[i] = h[i] - (h[i] - feq_dev(i, d, rho, ux, uy, uz, usq)) * inv_tau;
+    for (int p = 0; p < P; ++p) {
+        sum[p] = h[2 * p + 1] + h[2 * p + 2];
+        dif[p] = h[2 * p + 1] - h[2 * p + 2];
+    }
+    R d = h[0];
+#pragma unroll
+    for (int p = 0; p < P; ++p) d += sum[p];
+    // j = sum over pairs of c_i (h_i - h_{i+1}), nonzero components only
+    R j[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        R s = 0;
+        bool any = false;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const int c = lattice::D3Q19_C[2 * p + 1][a];
+            if (!c) continue;
+            const R t = c > 0 ? dif[p] : -dif[p];
+            s = any ? s + t : t;
+            any = true;
+        }
+        j[a] = s;
+    }
+    const R rho = (R)1 + d;
+    R ir;
+    if constexpr (sizeof(R) == 8)
+        ir = (R)1 / rho;
+    else
+        ir = __builtin_amdgcn_rcpf(rho);
+    const R ux = j[0] * ir, uy = j[1] * ir, uz = j[2] * ir;
+    const R usq = fmadd(ux, ux, fmadd(uy, uy, uz * uz));
+    const R c0 = fmadd((R)-1.5 * rho, usq, d);
+    const R r45 = (R)4.5 * rho;
+    const R om1 = (R)1 - inv_tau;
+    const R r3w = (R)3 * rho * inv_tau;
+    h[0] = fmadd(om1, h[0], (inv_tau * (R)wq(0)) * c0);
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const int i = 2 * p + 1;
+        const R cu = cdot(i, ux, uy, uz);
+        const R S = (inv_tau * (R)wq(i)) * fmadd(r45 * cu, cu, c0);
+        const R A = (r3w * (R)wq(i)) * cu;
+        h[i] = fmadd(om1, h[i], S + A);
+        h[i + 1] = fmadd(om1, h[i + 1], S - A);
+    }
 }
 
 // Per-cell link codes (2 bits per direction in a u64; bit 63 = active fluid

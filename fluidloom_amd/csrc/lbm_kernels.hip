@@ -95,21 +95,30 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
     using R = typename Store<T>::R;
     const int b = __builtin_amdgcn_readfirstlane(b_in);
     if (b >= L.nactive) return;
-    if constexpr (kOutlet) {
-        // only the bricks with an outlet link take the outlet code
-        if (!(L.bflags[b] & 2u)) {
-            collide_brick<T, false>(L, A, B, b);
-            return;
-        }
-    }
-    const int inst = L.binst[b];
-    if (!L.inst[inst].live) return;
-    const int lane = (int)(threadIdx.x & 63);
+    // What the population loads need comes from b alone: the neighbour row and
+    // the brick flags are one round of scalar loads.  The instance (its live
+    // flag and relaxation rate) only matters to the collision and the stores,
+    // so its two dependent loads overlap the population loads; a stopped
+    // instance keeps its layout, so reading its populations stays in bounds.
+    // binst is 16-bit: the wave-uniform dword holding it keeps the load on
+    // the scalar unit (device allocations are whole multiples of 256 B, so
+    // the dword's other half is inside the array).
     const int* __restrict__ row = L.nbr + (size_t)b * 27;
     int NB[27];
 #pragma unroll
     for (int k = 0; k < 27; ++k) NB[k] = row[k];
-    const bool simple = (L.bflags[b] & 1u) != 0;
+    const uint32_t bf = L.bflags[b];
+    const uint32_t ipair = reinterpret_cast<const uint32_t*>(L.binst)[b >> 1];
+    if constexpr (kOutlet) {
+        // only the bricks with an outlet link take the outlet code
+        if (!(bf & 2u)) {
+            collide_brick<T, false>(L, A, B, b);
+            return;
+        }
+    }
+    const int inst = (int)((b & 1) ? ipair >> 16 : ipair & 0xFFFFu);
+    const int lane = (int)(threadIdx.x & 63);
+    const bool simple = (bf & 1u) != 0;
     const unsigned slot = (unsigned)b * 64u + (unsigned)lane;
     const uint64_t w = simple ? 0ull : L.cflags[slot];
     const bool active = simple || (w & kActiveBit);
@@ -149,7 +158,7 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
             }
         }
     }
-    if (!active) return;
+    if (!active || !L.inst[inst].live) return;
     if (!kDevLoads) {
 #pragma unroll
         for (int i = 0; i < Q; ++i) f[i] -= (R)wq(i);
