@@ -143,6 +143,13 @@ struct Store<_Float16> {
         store_dev_sr(p, i, q, v - (float)wq(q), r);
     }
     __device__ static inline float dev(const _Float16* p, size_t i, int) { return (float)p[i]; }
+    // dev() at a 32-bit byte offset from a wave-uniform base: the load then
+    // takes the base in scalar registers and only the offset in a vector
+    // register (no 64-bit address per lane).  A plane group spans below 2^32
+    // bytes in 16-bit storage (bricks per level < 2^21, 1024 cells per brick).
+    __device__ static inline float dev_at(const _Float16* base, unsigned boff, int) {
+        return (float)*reinterpret_cast<const _Float16*>(reinterpret_cast<const char*>(base) + boff);
+    }
     __device__ static inline void store_dev(_Float16* p, size_t i, int, float h) { p[i] = (_Float16)h; }
     __device__ static inline void store_dev_sr(_Float16* p, size_t i, int, float h, uint32_t r) {
         typedef _Float16 h2 __attribute__((ext_vector_type(2)));
@@ -173,6 +180,11 @@ struct Store<fp16c> {
     // h = w x: the code float times 2^113 w
     __device__ static inline float dev(const fp16c* p, size_t i, int q) {
         return fp16c_code_float(p[i].bits) * ((float)wq(q) * kFp16cUp);
+    }
+    // dev() at a 32-bit byte offset from a wave-uniform base (as for _Float16)
+    __device__ static inline float dev_at(const fp16c* base, unsigned boff, int q) {
+        const uint16_t bits = *reinterpret_cast<const uint16_t*>(reinterpret_cast<const char*>(base) + boff);
+        return fp16c_code_float(bits) * ((float)wq(q) * kFp16cUp);
     }
     __device__ static inline void store_dev(fp16c* p, size_t i, int q, float h) {
         p[i].bits = fp16c_encode_rne(h * (float)(1.0 / wq(q)));
@@ -533,12 +545,15 @@ __device__ inline R special_link_dev(const LevelDev& L, const T* __restrict__ A,
 }
 
 // Offset of the source that cell (lx, ly, lz) of a brick pulls in direction
-// i, from plane i's base (A + plane_base(i, S)).  NB: the brick's 27
-// neighbour entries (13 = the brick itself); the source brick is one of at
-// most four of them, chosen by whether the pulled position leaves the brick
-// along each axis (static indices only, so NB stays in registers).  Bricks
-// per level are below 2^21 (the layout's limit): a 24-bit multiply (full
-// rate; the 3-plane group's stride 192 is not a power of two).
+// i, from plane i's base (A + plane_base(i, S)), in units of 1 / kScale
+// elements (kScale = 2 gives byte offsets into 16-bit storage, below 2^32).
+// NB: the brick's 27 neighbour entries (13 = the brick itself); the source
+// brick is one of at most four of them, chosen by whether the pulled
+// position leaves the brick along each axis (static indices only, so NB
+// stays in registers).  Bricks per level are below 2^21 (the layout's
+// limit): a 24-bit multiply (full rate; the 3-plane group's stride 192 is
+// not a power of two).
+template <unsigned kScale = 1>
 __device__ __attribute__((always_inline)) inline unsigned pull_offset(const int (&NB)[27], int lx, int ly, int lz,
                                                                       int i) {
     const int px = lx - cx(i), py = ly - cy(i), pz = lz - cz(i);
@@ -547,7 +562,8 @@ __device__ __attribute__((always_inline)) inline unsigned pull_offset(const int 
     const int n000 = NB[13], n100 = NB[13 + kx], n010 = NB[13 + ky], n110 = NB[13 + kx + ky];
     const int n001 = NB[13 + kz], n101 = NB[13 + kx + kz], n011 = NB[13 + ky + kz], n111 = NB[13 + kx + ky + kz];
     const int nb = ez ? (ey ? (ex ? n111 : n011) : (ex ? n101 : n001)) : (ey ? (ex ? n110 : n010) : (ex ? n100 : n000));
-    return __umul24((unsigned)nb, (unsigned)brick_stride(i)) + (unsigned)((px & 3) + 4 * (py & 3) + 16 * (pz & 3));
+    return __umul24((unsigned)nb, (unsigned)brick_stride(i) * kScale) +
+           kScale * (unsigned)((px & 3) + 4 * (py & 3) + 16 * (pz & 3));
 }
 
 // Generic pull of the 19 incoming populations of one slot (list kernels).

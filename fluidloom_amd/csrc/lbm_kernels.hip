@@ -126,6 +126,14 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
     const size_t S = L.S;
     // pulled source of direction i: offset from plane i's base (A + plane_base)
     auto source = [&](int i) -> unsigned { return pull_offset(NB, lx, ly, lz, i); };
+    // pulled deviation of direction i in 16-bit storage: byte offsets from a
+    // scalar plane base
+    auto dev16 = [&](int i) -> R {
+        if constexpr (sizeof(T) == 2)
+            return Store<T>::dev_at(A + plane_base(i, S), pull_offset<2>(NB, lx, ly, lz, i), i);
+        else
+            return (R)0;
+    };
     // The cell's populations as deviations h = f - w: density and momentum
     // then come from sums of small numbers, so neither the weights' rounding
     // to the storage type nor the low bits that 16-bit storage leaves fixed
@@ -140,7 +148,7 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
         // loads back to back.  FP32 and FP64 keep the interleaved form below,
         // which measured faster for FP32.
 #pragma unroll
-        for (int i = 0; i < Q; ++i) f[i] = Store<T>::dev(A + plane_base(i, S), source(i), i);
+        for (int i = 0; i < Q; ++i) f[i] = dev16(i);
     } else {
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
@@ -148,8 +156,7 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
             if (active) {
                 const unsigned code = (unsigned)(w >> (2 * i)) & 3u;
                 if (code == kLinkRegular)
-                    f[i] = kDevLoads ? Store<T>::dev(A + plane_base(i, S), source(i), i)
-                                     : Store<T>::load(A + plane_base(i, S), source(i), i);
+                    f[i] = kDevLoads ? dev16(i) : Store<T>::load(A + plane_base(i, S), source(i), i);
                 else
                     f[i] = kDevLoads ? special_link_dev<T, R, kOutlet>(L, A, b, slot, i, code, lx - cx(i), ly - cy(i),
                                                                        lz - cz(i))
@@ -180,10 +187,23 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
                                         mix32((uint32_t)c3[2] * 0xC2B2AE3Du ^ ((uint32_t)L.level << 24) ^
                                               (*L.tick * 0x27D4EB2Fu))));
         const uint32_t h = mix32(hb + (uint32_t)lane);
+        if constexpr (std::is_same<T, _Float16>::value) {
+            // v_cvt_sr_f16_f32 writes one half of its register and keeps the
+            // other: converting every direction into the same register keeps
+            // a fresh zeroed register (one move) per direction out
+            typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+            h2 acc = {(_Float16)0, (_Float16)0};
 #pragma unroll
-        for (int i = 0; i < Q; ++i)
-            Store<T>::store_dev_sr(B + plane_base(i, S), (unsigned)b * (unsigned)brick_stride(i) + (unsigned)lane, i,
-                                   f[i], h + (uint32_t)i * 0x9E3779B9u);
+            for (int i = 0; i < Q; ++i) {
+                acc = __builtin_amdgcn_cvt_sr_f16_f32(acc, f[i], h + (uint32_t)i * 0x9E3779B9u, false);
+                __builtin_nontemporal_store(acc[0], B + plane_base(i, S) + (size_t)b * brick_stride(i) + lane);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < Q; ++i)
+                Store<T>::store_dev_sr(B + plane_base(i, S), (unsigned)b * (unsigned)brick_stride(i) + (unsigned)lane,
+                                       i, f[i], h + (uint32_t)i * 0x9E3779B9u);
+        }
     } else {
 #pragma unroll
         for (int i = 0; i < Q; ++i)
