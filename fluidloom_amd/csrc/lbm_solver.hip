@@ -212,6 +212,9 @@ struct DevLevel {
     float* uacc = nullptr;    // [nactive * 64 * 3] sum of the window's velocity samples (first use)
     void* buf[2] = {nullptr, nullptr};
     void* cap[2] = {nullptr, nullptr};
+    // cap[0] holds this level's capture of buf[1 - cur] from the end of the
+    // previous step of the current stepping pass (see advance)
+    bool cap_fresh = false;
     int cur = 0;
     LevelDev ld{};
 };
@@ -907,7 +910,13 @@ fl_status advance(fl_lbm_solver* S, int l) {
         DevLevel& F = S->d[l + 1];
         FL_HIP(launch_fill_f2c<T>(D.ld, D.f2c_ghost, D.f2c_child, (uint32_t)H.n_f2c, F.ld,
                                   static_cast<const T*>(F.buf[1 - F.cur]), static_cast<T*>(D.buf[D.cur]), st));
-        FL_HIP(launch_capture<T>(D.ld, static_cast<const T*>(D.buf[1 - D.cur]), D.cap_slots, ncap, D.cap[0], st));
+        // The capture before the collide gathers from buf[1 - cur]: the
+        // buffer the previous step's capture after its collide gathered from,
+        // unchanged since (the f2c fill above writes buf[cur], the finer
+        // levels write only their own buffers).  So within a pass only the
+        // first step captures it; later steps take the previous capture.
+        if (!D.cap_fresh)
+            FL_HIP(launch_capture<T>(D.ld, static_cast<const T*>(D.buf[1 - D.cur]), D.cap_slots, ncap, D.cap[0], st));
     }
     FL_HIP(launch_collide<T>(D.ld, static_cast<const T*>(D.buf[D.cur]), static_cast<T*>(D.buf[1 - D.cur]),
                              S->variant[l], st));
@@ -923,8 +932,19 @@ fl_status advance(fl_lbm_solver* S, int l) {
             fl_status r = advance<T>(S, l + 1);
             if (r) return r;
         }
+        // this step's capture after the collide is the next step's before it
+        std::swap(D.cap[0], D.cap[1]);
+        D.cap_fresh = true;
     }
     return FL_OK;
+}
+
+// Start of a stepping pass (an eager run of steps or a graph capture): every
+// level captures afresh at its first step, so a pass never depends on
+// captures made before it (a graph replays the same way whatever ran
+// between its launches).
+void begin_pass(fl_lbm_solver* S) {
+    for (int l = 0; l < S->L; ++l) S->d[l].cap_fresh = false;
 }
 
 fl_status advance_any(fl_lbm_solver* S) {
@@ -940,6 +960,7 @@ fl_status advance_any(fl_lbm_solver* S) {
 // parity is back where it started; levels >= 1 flip an even number of times
 // per step, so only level 0's parity selects the graph.
 fl_status capture_graph(fl_lbm_solver* S, int k) {
+    begin_pass(S);
     FL_HIP(hipStreamBeginCapture(S->stream, hipStreamCaptureModeRelaxed));
     fl_status r = advance_any(S);
     if (!r) r = advance_any(S);
@@ -974,6 +995,7 @@ fl_status dispatch_step(fl_lbm_solver* S, int nsteps) {
             ++S->graph_launches;
         }
     }
+    if (t < nsteps) begin_pass(S);
     for (; t < nsteps; ++t) {
         fl_status r = advance_any(S);
         if (r) return r;
