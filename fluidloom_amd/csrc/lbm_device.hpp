@@ -566,6 +566,36 @@ __device__ __attribute__((always_inline)) inline unsigned pull_offset(const int 
            kScale * (unsigned)((px & 3) + 4 * (py & 3) + 16 * (pz & 3));
 }
 
+// The neighbour-row entries one cell can pull from: its own brick and the
+// seven bricks on its octant's side.  A cell with lx < 2 leaves the brick
+// only towards -x (lx = 0 pulling along +x), one with lx >= 2 only towards
+// +x; likewise along y and z.  o[dz][dy][dx] = row[13 + dx sx + 3 dy sy +
+// 9 dz sz] with s = -1 / +1 by octant: 8 loads in place of the row's 27.
+struct Octant {
+    int o000, o001, o010, o011, o100, o101, o110, o111;
+};
+__device__ __attribute__((always_inline)) inline Octant octant_row(const int* __restrict__ row, int lx, int ly,
+                                                                   int lz) {
+    const int sx = lx < 2 ? -1 : 1, sy = ly < 2 ? -3 : 3, sz = lz < 2 ? -9 : 9;
+    return Octant{row[13],      row[13 + sx],      row[13 + sy],      row[13 + sy + sx],
+                  row[13 + sz], row[13 + sz + sx], row[13 + sz + sy], row[13 + sz + sy + sx]};
+}
+
+// pull_offset from the octant's row entries (octant_row): a pulled position
+// that leaves the brick along an axis leaves it on the octant's side.
+__device__ __attribute__((always_inline)) inline unsigned pull_offset_oct(const Octant& O, int lx, int ly, int lz,
+                                                                          int i) {
+    const int px = lx - cx(i), py = ly - cy(i), pz = lz - cz(i);
+    const bool ex = (unsigned)px > 3u, ey = (unsigned)py > 3u, ez = (unsigned)pz > 3u;
+    // select between values, not between the fields themselves: a choice of
+    // lvalues becomes a load through a chosen address, which keeps the
+    // struct in scratch memory
+    const int a000 = O.o000, a001 = O.o001, a010 = O.o010, a011 = O.o011;
+    const int a100 = O.o100, a101 = O.o101, a110 = O.o110, a111 = O.o111;
+    const int nb = ez ? (ey ? (ex ? a111 : a110) : (ex ? a101 : a100)) : (ey ? (ex ? a011 : a010) : (ex ? a001 : a000));
+    return __umul24((unsigned)nb, (unsigned)brick_stride(i)) + (unsigned)((px & 3) + 4 * (py & 3) + 16 * (pz & 3));
+}
+
 // Generic pull of the 19 incoming populations of one slot (list kernels).
 // w: the slot's link word (L.cflags[slot]), which callers may load early.
 template <typename T, typename R>
@@ -575,14 +605,11 @@ __device__ inline void gather_slot(const LevelDev& L, const T* __restrict__ A, i
     const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
     if ((w & ((1ull << (2 * Q)) - 1)) == 0) {
         // every link regular (the common case, e.g. the children at a level
-        // interface): the brick's neighbour row first, then 19 independent
-        // population loads (a regular link's source brick exists)
-        const int* __restrict__ row = L.nbr + (size_t)b * 27;
-        int NB[27];
+        // interface): the cell's octant of the neighbour row first, then 19
+        // independent population loads (a regular link's source brick exists)
+        const Octant O = octant_row(L.nbr + (size_t)b * 27, lx, ly, lz);
 #pragma unroll
-        for (int k = 0; k < 27; ++k) NB[k] = row[k];
-#pragma unroll
-        for (int i = 0; i < Q; ++i) f[i] = Store<T>::load(A + plane_base(i, L.S), pull_offset(NB, lx, ly, lz, i), i);
+        for (int i = 0; i < Q; ++i) f[i] = Store<T>::load(A + plane_base(i, L.S), pull_offset_oct(O, lx, ly, lz, i), i);
         return;
     }
 #pragma unroll
@@ -608,12 +635,9 @@ __device__ inline void gather_slot_dev(const LevelDev& L, const T* __restrict__ 
     const int l = (int)(slot & 63);
     const int lx = l & 3, ly = (l >> 2) & 3, lz = l >> 4;
     if ((w & ((1ull << (2 * Q)) - 1)) == 0) {
-        const int* __restrict__ row = L.nbr + (size_t)b * 27;
-        int NB[27];
+        const Octant O = octant_row(L.nbr + (size_t)b * 27, lx, ly, lz);
 #pragma unroll
-        for (int k = 0; k < 27; ++k) NB[k] = row[k];
-#pragma unroll
-        for (int i = 0; i < Q; ++i) h[i] = Store<T>::dev(A + plane_base(i, L.S), pull_offset(NB, lx, ly, lz, i), i);
+        for (int i = 0; i < Q; ++i) h[i] = Store<T>::dev(A + plane_base(i, L.S), pull_offset_oct(O, lx, ly, lz, i), i);
         return;
     }
 #pragma unroll
