@@ -404,7 +404,9 @@ struct LevelDev {
     const int32_t* nbr;      // [brick][27], -1 = none
     const int32_t* bcoord;   // [brick][3]
     const uint32_t* bflags;  // bit 0: simple brick (all 64 active, all links regular);
-                             // bit 1: a cell of the brick has a pressure-outlet link
+                             // bit 1: a cell of the brick has a pressure-outlet link;
+                             // bit 2: a 2x2x2 block of the brick is the children of a
+                             // staged f2c ghost of the coarser level (f2c_blk)
     const uint64_t* cflags;  // [slot]
     const uint16_t* binst;   // [brick] instance id
     const float* itau;       // [instance] 1 / tau at this level
@@ -415,6 +417,15 @@ struct LevelDev {
     int32_t nactive;         // active bricks (they come first)
     int32_t level;
     int32_t outlet;          // 1 when a fluid leaf of the level has a pressure-outlet link
+    // f2c staging (this level's collide feeding the coarser level's f2c fill):
+    // f2c_blk[brick * 8 + block] = index of the coarse f2c ghost whose children
+    // are that 2x2x2 block (kNoCap: none); when stage_on, the collide writes
+    // those ghosts' child sums to stage ([23][n_stage], the coarse level's
+    // Store<T>::R), block = (lx >> 1) + 2 (ly >> 1) + 4 (lz >> 1)
+    const uint32_t* f2c_blk;
+    void* stage;
+    uint32_t n_stage;
+    int32_t stage_on;
 };
 
 __host__ __device__ inline double tau_at(double tau0, int level) {

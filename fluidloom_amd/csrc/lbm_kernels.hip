@@ -83,6 +83,58 @@ __device__ inline R inv_tau_of(const LevelDev& L, int inst) {
     return (R)L.itau[inst];
 }
 
+// v of the lane whose index differs in the bits of m (m < 32): ds_swizzle in
+// its bit mode (and 0x1F, xor m), no lane-index arithmetic
+template <int M, typename R>
+__device__ __attribute__((always_inline)) inline R swz_xor(R v) {
+    constexpr int kPat = (M << 10) | 0x1F;
+    if constexpr (sizeof(R) == 8) {
+        const long long x = __builtin_bit_cast(long long, v);
+        const int lo = __builtin_amdgcn_ds_swizzle((int)(x & 0xFFFFFFFFll), kPat);
+        const int hi = __builtin_amdgcn_ds_swizzle((int)(x >> 32), kPat);
+        return __builtin_bit_cast(R, ((long long)hi << 32) | (long long)(unsigned)lo);
+    } else {
+        return __builtin_bit_cast(R, __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, v), kPat));
+    }
+}
+
+// f2c staging, run by the collide of a fine level's second sub-step on the
+// bricks that hold children of staged coarse f2c ghosts (bflags bit 2).  h:
+// the cell's pulled deviations, the values the coarse level's f2c fill would
+// gather from this collide's input at its next step (nothing writes that
+// buffer in between).  Each child contributes (rho - 1, momentum, f_neq) as
+// in k_fill_f2c; the 2x2x2 block's sum is formed across lanes 1, 4 and 16
+// apart (x, then y, then z: group8_sum's tree, so the same bits), and the
+// block's eight lanes store the 23 sums of their ghost.  Staged children
+// have regular links only (k_stage_map), so their pulled values are the
+// fill's gather bit for bit.  The exchanges are
+// ds_swizzle: __shfl_xor's lane-index arithmetic here made the 16-bit
+// collide instances allocate all 256 registers.  Every lane of a
+// staged block is an active fluid leaf (the map admits no other), so the
+// shuffles within the block read live lanes.
+template <typename R>
+__device__ __attribute__((always_inline)) inline void stage_children(const LevelDev& L, const R* h, int b, int lane) {
+    const int blk = ((lane >> 1) & 1) | ((lane >> 2) & 2) | ((lane >> 3) & 4);
+    const int q = (lane & 1) | ((lane >> 1) & 2) | ((lane >> 2) & 4);
+    const uint32_t k = L.f2c_blk[(size_t)b * 8 + blk];
+    R d, jx, jy, jz;
+    dev_density_momentum(h, d, jx, jy, jz);
+    const R rho = (R)1 + d;
+    const R ir = (R)1 / rho;
+    const R vx = jx * ir, vy = jy * ir, vz = jz * ir;
+    const R usq = vx * vx + vy * vy + vz * vz;
+    R* __restrict__ st = static_cast<R*>(L.stage);
+#pragma unroll
+    for (int j = 0; j < 4 + Q; ++j) {
+        R v = j == 0 ? d : j == 1 ? jx : j == 2 ? jy : j == 3 ? jz
+                     : h[j - 4] - feq_dev(j - 4, d, rho, vx, vy, vz, usq);
+        v += swz_xor<1>(v);
+        v += swz_xor<4>(v);
+        v += swz_xor<16>(v);
+        if (k != kNoCap && (j & 7) == q) st[(size_t)j * L.n_stage + k] = v;
+    }
+}
+
 // One wave per brick, lane = cell.  The brick's 27 neighbour entries are
 // wave-uniform loads (scalar registers); for direction i a lane's source
 // brick is one of at most four of them, chosen by whether the pulled
@@ -170,6 +222,7 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
 #pragma unroll
         for (int i = 0; i < Q; ++i) f[i] -= (R)wq(i);
     }
+    if (L.stage_on && (bf & 4u)) stage_children<R>(L, f, b, lane);
     bgk_dev(f, inv_tau_of<R>(L, inst));
     if constexpr (std::is_same<T, _Float16>::value || std::is_same<T, fp16c>::value) {
         // rounding bits from the cell's own coordinates, level and step, so an
@@ -256,6 +309,11 @@ __global__ __launch_bounds__(kWG) void k_any_outlet(const uint64_t* __restrict__
         any = any || o;
     }
     if (__ballot(any) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
+__global__ __launch_bounds__(kWG) void k_fill_u32_kernel(uint32_t* __restrict__ p, size_t n, uint32_t v) {
+    const size_t i = (size_t)blockIdx.x * kWG + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 __global__ void k_tick(uint32_t* tick) {
@@ -457,45 +515,59 @@ __device__ inline void child_average(const LevelDev& Lf, const T* __restrict__ A
 template <typename T>
 __global__ __launch_bounds__(kWG) void k_fill_f2c(LevelDev Lc, const uint32_t* __restrict__ ghost,
                                                   const uint32_t* __restrict__ child8, uint32_t n, LevelDev Lf,
-                                                  const T* __restrict__ Af, T* __restrict__ Ac) {
+                                                  const T* __restrict__ Af, T* __restrict__ Ac,
+                                                  const typename Store<T>::R* __restrict__ stage,
+                                                  const uint8_t* __restrict__ staged) {
     using R = typename Store<T>::R;
     const uint32_t t = blockIdx.x * kWG + threadIdx.x;
     const uint32_t k = t >> 3;
     const int q = (int)(t & 7u);
-    // every lane of a group shares k, so a group is wholly in or out
+    // every lane of a group shares k, so a group is wholly in or out, and
+    // wholly staged or not
     bool on = k < n;
+    bool stg = false;
     size_t g = 0;
     uint32_t child = 0;
     uint64_t w = 0;
     if (on) {
         g = ghost[k];
-        child = child8[(size_t)8 * k + q];
-        w = Lf.cflags[child];
+        stg = stage && staged[k];
+        if (!stg) {
+            child = child8[(size_t)8 * k + q];
+            w = Lf.cflags[child];
+        }
         on = Lc.inst[Lc.binst[g >> 6]].live != 0;
     }
     R s[4 + Q];
 #pragma unroll
     for (int j = 0; j < 4 + Q; ++j) s[j] = 0;
-    if (on) {
-        // the child's deviations h = f - w: s = (rho - 1, momentum, f_neq)
-        R f[Q];
-        gather_slot_dev<T, R>(Lf, Af, child, w, f);
-        R d, jx, jy, jz;
-        dev_density_momentum(f, d, jx, jy, jz);
-        const R rho = (R)1 + d;
-        const R ir = (R)1 / rho;
-        const R vx = jx * ir, vy = jy * ir, vz = jz * ir;
-        const R usq = vx * vx + vy * vy + vz * vz;
-        s[0] = d;
-        s[1] = jx;
-        s[2] = jy;
-        s[3] = jz;
+    if (stg) {
+        // the group sums the fine collide formed (stage_children), the same bits
 #pragma unroll
-        for (int i = 0; i < Q; ++i) s[4 + i] = f[i] - feq_dev(i, d, rho, vx, vy, vz, usq);
+        for (int j = 0; j < 4 + Q; ++j) s[j] = stage[(size_t)j * n + k];
+    } else {
+        if (on) {
+            // the child's deviations h = f - w: s = (rho - 1, momentum, f_neq)
+            R f[Q];
+            gather_slot_dev<T, R>(Lf, Af, child, w, f);
+            R d, jx, jy, jz;
+            dev_density_momentum(f, d, jx, jy, jz);
+            const R rho = (R)1 + d;
+            const R ir = (R)1 / rho;
+            const R vx = jx * ir, vy = jy * ir, vz = jz * ir;
+            const R usq = vx * vx + vy * vy + vz * vz;
+            s[0] = d;
+            s[1] = jx;
+            s[2] = jy;
+            s[3] = jz;
+#pragma unroll
+            for (int i = 0; i < Q; ++i) s[4 + i] = f[i] - feq_dev(i, d, rho, vx, vy, vz, usq);
+        }
+        // every lane of the group is active here (whole waves, no lane has
+        // left, the branch is group-uniform); the sum stays within the group
+#pragma unroll
+        for (int j = 0; j < 4 + Q; ++j) s[j] = group8_sum(s[j]);
     }
-    // the grid has whole waves and no lane has left: every lane is active
-#pragma unroll
-    for (int j = 0; j < 4 + Q; ++j) s[j] = group8_sum(s[j]);
     if (!on) return;
     const R fac = (R)f2c_factor(Lc.inst[Lc.binst[g >> 6]].tau0, Lc.level);
     const R dm = s[0] * (R)0.125;
@@ -1024,11 +1096,53 @@ hipError_t launch_fill_c2f(const LevelDev& Lf, const uint32_t* gchild, const uin
 
 template <typename T>
 hipError_t launch_fill_f2c(const LevelDev& Lc, const uint32_t* ghost, const uint32_t* child8, uint32_t n,
-                           const LevelDev& Lf, const T* Af, T* Ac, hipStream_t s) {
+                           const LevelDev& Lf, const T* Af, T* Ac, const void* stage, const uint8_t* staged,
+                           hipStream_t s) {
     if (n == 0) return hipSuccess;
+    using R = typename Store<T>::R;
     // eight lanes per ghost, one pass (no grid stride: a group must stay whole)
     const size_t blocks = ((size_t)n * 8 + kWG - 1) / kWG;
-    hipLaunchKernelGGL(k_fill_f2c<T>, dim3((unsigned)blocks), dim3(kWG), 0, s, Lc, ghost, child8, n, Lf, Af, Ac);
+    hipLaunchKernelGGL(k_fill_f2c<T>, dim3((unsigned)blocks), dim3(kWG), 0, s, Lc, ghost, child8, n, Lf, Af, Ac,
+                       static_cast<const R*>(stage), staged);
+    return hipGetLastError();
+}
+
+// The staging map of fine level Lf for the coarse level's f2c ghosts: ghost
+// k is staged when its eight children are active fluid leaves of one brick
+// (they form a 2x2x2 block of it) whose links are all regular pulls; then
+// f2c_blk[brick * 8 + block] = k and the brick's bflags gain bit 2.  (A wall
+// or outlet link's value is formed from the population in the FP32 collide
+// and from the deviation in the fill's gather, which round differently; a
+// regular pull is the same load either way.)  f2c_blk starts all kNoCap.
+__global__ __launch_bounds__(kWG) void k_stage_map(const uint32_t* __restrict__ child8, uint32_t n,
+                                                   const uint64_t* __restrict__ cflags, uint32_t* __restrict__ bflags,
+                                                   uint32_t* __restrict__ f2c_blk, uint8_t* __restrict__ staged) {
+    const uint32_t k = blockIdx.x * kWG + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t c0 = child8[(size_t)8 * k];
+    bool ok = (c0 & 0x15u) == 0;  // the block's even corner
+    for (int q = 0; q < 8 && ok; ++q) {
+        const uint32_t c = child8[(size_t)8 * k + q];
+        const uint64_t w = cflags[c];
+        ok = (c >> 6) == (c0 >> 6) && (w & kActiveBit) != 0 && (w & ((1ull << (2 * Q)) - 1)) == 0;
+    }
+    staged[k] = ok ? 1 : 0;
+    if (!ok) return;
+    const uint32_t l = c0 & 63u;
+    const uint32_t blk = ((l >> 1) & 1u) | ((l >> 2) & 2u) | ((l >> 3) & 4u);
+    f2c_blk[(size_t)(c0 >> 6) * 8 + blk] = k;
+    atomicOr(bflags + (c0 >> 6), 4u);
+}
+
+hipError_t launch_stage_map(const uint32_t* child8, uint32_t n, const uint64_t* cflags, uint32_t* bflags,
+                            uint32_t* f2c_blk, size_t nbricks, uint8_t* staged, hipStream_t s) {
+    if (nbricks) {
+        const size_t m = nbricks * 8;
+        hipLaunchKernelGGL(k_fill_u32_kernel, dim3((unsigned)((m + kWG - 1) / kWG)), dim3(kWG), 0, s, f2c_blk, m,
+                           kNoCap);
+    }
+    if (n) hipLaunchKernelGGL(k_stage_map, dim3((n + kWG - 1) / kWG), dim3(kWG), 0, s, child8, n, cflags, bflags,
+                              f2c_blk, staged);
     return hipGetLastError();
 }
 
@@ -1145,7 +1259,8 @@ hipError_t launch_ghost_eq(const uint32_t* c2f_gchild, const uint32_t* c2f_gcap,
     template hipError_t launch_fill_c2f<T>(const LevelDev&, const uint32_t*, const uint32_t*, const uint32_t*,  \
                                            uint32_t, const void*, const void*, uint32_t, double, T*, hipStream_t); \
     template hipError_t launch_fill_f2c<T>(const LevelDev&, const uint32_t*, const uint32_t*, uint32_t,         \
-                                           const LevelDev&, const T*, T*, hipStream_t);                         \
+                                           const LevelDev&, const T*, T*, const void*, const uint8_t*,          \
+                                           hipStream_t);                                                        \
     template hipError_t launch_init<T>(const LevelDev&, T*, T*, hipStream_t, bool);                             \
     template hipError_t launch_brick_sums<T>(const LevelDev&, const T*, double, double*, hipStream_t);          \
     template hipError_t launch_brick_du<T>(const LevelDev&, const T*, float*, float*, float, double*, hipStream_t); \

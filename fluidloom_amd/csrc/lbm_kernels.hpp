@@ -28,9 +28,16 @@ template <typename T>
 hipError_t launch_fill_c2f(const LevelDev& Lf, const uint32_t* gchild, const uint32_t* gcap, const uint32_t* gmask,
                            uint32_t ngroups, const void* cap0, const void* cap1, uint32_t ncap, double alpha, T* A,
                            hipStream_t s);
+// stage, staged: the child sums the fine collide staged (LevelDev::stage) and
+// which ghosts have them; stage == nullptr gathers every ghost's children
 template <typename T>
 hipError_t launch_fill_f2c(const LevelDev& Lc, const uint32_t* ghost, const uint32_t* child8, uint32_t n,
-                           const LevelDev& Lf, const T* Af, T* Ac, hipStream_t s);
+                           const LevelDev& Lf, const T* Af, T* Ac, const void* stage, const uint8_t* staged,
+                           hipStream_t s);
+// f2c staging map of a fine level (nbricks bricks) for the coarse level's n
+// f2c ghosts (k_stage_map)
+hipError_t launch_stage_map(const uint32_t* child8, uint32_t n, const uint64_t* cflags, uint32_t* bflags,
+                            uint32_t* f2c_blk, size_t nbricks, uint8_t* staged, hipStream_t s);
 template <typename T>
 hipError_t launch_init(const LevelDev& L, T* A, T* B, hipStream_t s, bool skip_simple);
 // FP64 reductions: per-brick (or per-link) values, then one workgroup per

@@ -215,6 +215,15 @@ struct DevLevel {
     // cap[0] holds this level's capture of buf[1 - cur] from the end of the
     // previous step of the current stepping pass (see advance)
     bool cap_fresh = false;
+    // f2c staging: this (coarse) level's staged child sums, [23][n_f2c] in
+    // the compute type, and which ghosts have them (LevelDev::stage); as a
+    // fine level, its bricks' block -> coarse ghost map (LevelDev::f2c_blk).
+    // stage_fresh: the finer level's last collide staged this level's
+    // ghosts in the current pass.
+    void* f2c_stage = nullptr;
+    uint8_t* f2c_staged = nullptr;
+    uint32_t* f2c_blk = nullptr;
+    bool stage_fresh = false;
     int cur = 0;
     LevelDev ld{};
 };
@@ -893,11 +902,32 @@ fl_status upload_and_init(fl_lbm_solver* S, bool tables_on_device, bool remap_fo
     FL_HIP(hipMemcpyAsync(hf.data(), flags, (size_t)S->L * sizeof(uint32_t), hipMemcpyDeviceToHost, S->stream));
     FL_HIP(hipStreamSynchronize(S->stream));
     for (int l = 0; l < S->L; ++l) S->d[l].ld.outlet = hf[(size_t)l] ? 1 : 0;
+    // f2c staging: which coarse f2c ghosts the finer level's collide can
+    // stage (children all active fluid leaves of one brick), per fine brick
+    // and block
+    for (int l = 0; l + 1 < S->L; ++l) {
+        const HostLevel& H = S->h[l];
+        DevLevel& D = S->d[l];
+        DevLevel& F = S->d[l + 1];
+        if (H.n_f2c == 0) continue;
+        const size_t nbf = (size_t)S->h[l + 1].nb;
+        if ((r = dev_alloc(S, &D.f2c_stage, (size_t)(4 + Q) * H.n_f2c * compute_bytes(S->storage)))) return r;
+        if ((r = dev_alloc(S, reinterpret_cast<void**>(&D.f2c_staged), H.n_f2c))) return r;
+        if ((r = dev_alloc(S, reinterpret_cast<void**>(&F.f2c_blk), nbf * 8 * sizeof(uint32_t)))) return r;
+        FL_HIP(launch_stage_map(D.f2c_child, (uint32_t)H.n_f2c, F.cflags, F.bflags, F.f2c_blk, nbf, D.f2c_staged,
+                                S->stream));
+        F.ld.f2c_blk = F.f2c_blk;
+        F.ld.stage = D.f2c_stage;
+        F.ld.n_stage = (uint32_t)H.n_f2c;
+        F.ld.stage_on = 0;
+    }
     return FL_OK;
 }
 
+// stage_sub: this step is the second sub-step of the coarser level's step,
+// whose collide stages the coarser level's f2c ghosts (LevelDev::stage).
 template <typename T>
-fl_status advance(fl_lbm_solver* S, int l) {
+fl_status advance(fl_lbm_solver* S, int l, bool stage_sub) {
     DevLevel& D = S->d[l];
     const HostLevel& H = S->h[l];
     // levels are advanced while any deeper level holds fluid leaves (a mesh
@@ -908,8 +938,13 @@ fl_status advance(fl_lbm_solver* S, int l) {
     const uint32_t ncap = (uint32_t)H.n_cap;
     if (fine) {
         DevLevel& F = S->d[l + 1];
+        // The f2c fill gathers the children's pulled populations from the
+        // input of the finer level's last collide, which (from the second
+        // step of a pass on) staged their sums for the ghosts it could
+        // (nothing writes that input in between): those ghosts read the sums.
         FL_HIP(launch_fill_f2c<T>(D.ld, D.f2c_ghost, D.f2c_child, (uint32_t)H.n_f2c, F.ld,
-                                  static_cast<const T*>(F.buf[1 - F.cur]), static_cast<T*>(D.buf[D.cur]), st));
+                                  static_cast<const T*>(F.buf[1 - F.cur]), static_cast<T*>(D.buf[D.cur]),
+                                  D.stage_fresh ? D.f2c_stage : nullptr, D.f2c_staged, st));
         // The capture before the collide gathers from buf[1 - cur]: the
         // buffer the previous step's capture after its collide gathered from,
         // unchanged since (the f2c fill above writes buf[cur], the finer
@@ -918,7 +953,9 @@ fl_status advance(fl_lbm_solver* S, int l) {
         if (!D.cap_fresh)
             FL_HIP(launch_capture<T>(D.ld, static_cast<const T*>(D.buf[1 - D.cur]), D.cap_slots, ncap, D.cap[0], st));
     }
-    FL_HIP(launch_collide<T>(D.ld, static_cast<const T*>(D.buf[D.cur]), static_cast<T*>(D.buf[1 - D.cur]),
+    LevelDev ld = D.ld;
+    ld.stage_on = stage_sub && ld.stage ? 1 : 0;
+    FL_HIP(launch_collide<T>(ld, static_cast<const T*>(D.buf[D.cur]), static_cast<T*>(D.buf[1 - D.cur]),
                              S->variant[l], st));
     if (S->storage == FL_STORE_F16 || S->storage == FL_STORE_F16C) FL_HIP(launch_tick(S->ticks + l, st));
     D.cur ^= 1;
@@ -929,12 +966,13 @@ fl_status advance(fl_lbm_solver* S, int l) {
         for (int sub = 0; sub < 2; ++sub) {
             FL_HIP(launch_fill_c2f<T>(F.ld, F.c2f_gchild, F.c2f_gcap, F.c2f_gmask, (uint32_t)HF.n_groups, D.cap[0], D.cap[1],
                                       ncap, 0.5 * sub, static_cast<T*>(F.buf[F.cur]), st));
-            fl_status r = advance<T>(S, l + 1);
+            fl_status r = advance<T>(S, l + 1, sub == 1);
             if (r) return r;
         }
         // this step's capture after the collide is the next step's before it
         std::swap(D.cap[0], D.cap[1]);
         D.cap_fresh = true;
+        D.stage_fresh = D.f2c_stage != nullptr;
     }
     return FL_OK;
 }
@@ -944,15 +982,18 @@ fl_status advance(fl_lbm_solver* S, int l) {
 // captures made before it (a graph replays the same way whatever ran
 // between its launches).
 void begin_pass(fl_lbm_solver* S) {
-    for (int l = 0; l < S->L; ++l) S->d[l].cap_fresh = false;
+    for (int l = 0; l < S->L; ++l) {
+        S->d[l].cap_fresh = false;
+        S->d[l].stage_fresh = false;
+    }
 }
 
 fl_status advance_any(fl_lbm_solver* S) {
     switch (S->storage) {
-        case FL_STORE_F32: return advance<float>(S, 0);
-        case FL_STORE_F16: return advance<_Float16>(S, 0);
-        case FL_STORE_F16C: return advance<fp16c>(S, 0);
-        default: return advance<double>(S, 0);
+        case FL_STORE_F32: return advance<float>(S, 0, false);
+        case FL_STORE_F16: return advance<_Float16>(S, 0, false);
+        case FL_STORE_F16C: return advance<fp16c>(S, 0, false);
+        default: return advance<double>(S, 0, false);
     }
 }
 

@@ -77,6 +77,28 @@ def test_graph_replay_equals_eager(fl, oracle):
     assert g.stats().graph == 1 and g.stats().graph_launches >= 6
 
 
+@pytest.mark.parametrize("storage", ["f32", "f16", "f16c", "f64"])
+def test_staged_f2c_equals_gathered(fl, oracle, storage):
+    # From the second step of a stepping pass on, the f2c fills read the child
+    # sums that the finer level's collide staged; a pass's first step gathers
+    # them from the fine populations.  Six steps in one pass against six
+    # single-step passes (which never stage at the coarsest level): the same
+    # bits.
+    names = ["sphere3", "cavity2", "patch2"]
+    _, a = batch_of(oracle, names, storage)
+    _, b = batch_of(oracle, names, storage)
+    for k in range(len(names)):
+        f0 = perturb(a.get_f(k), 41 + k, 1e-3)
+        a.set_f(f0, inst=k)
+        b.set_f(f0, inst=k)
+    a.step(6)
+    for _ in range(6):
+        b.step(1)
+    for k in range(len(names)):
+        assert np.array_equal(a.get_f(k), b.get_f(k)), names[k]
+    assert np.array_equal(a.totals_batch()[0], b.totals_batch()[0])
+
+
 def test_collide_variants_agree(fl, oracle):
     from fluidloom_amd.solver import variant_names
     names = ["sphere3", "patch2"]
