@@ -468,12 +468,23 @@ __global__ __launch_bounds__(kWG) void k_fill_c2f(LevelDev Lf, const uint32_t* _
     const R rphi = (R)1 + phi[0];
     // only the populations some fluid leaf pulls from this ghost
     const uint32_t rb = slot_bits(Lf, g);
+    auto put = [&](int i, R e, R em) {
+        if ((want >> i) & 1u) store_ghost<T>(A, pidx(i, Lf.S, g), i, e + fac * (f[i] - em), rb);
+    };
+    // Equilibrium deviations (feq_dev) of the interpolated state (phi) and of
+    // P's own over the opposite pairs (i, i+1), which share |c_i . u| and the
+    // weight: feq_dev = w (c0 + 4.5 rho cu^2) +- 3 w rho cu, c0 = d - 1.5 rho u^2.
+    const R c0p = fmadd((R)-1.5 * rphi, usq, phi[0]), r45p = (R)4.5 * rphi, r3p = (R)3 * rphi;
+    const R c0m = fmadd((R)-1.5 * rp, usp, mp[0]), r45m = (R)4.5 * rp, r3m = (R)3 * rp;
+    put(0, (R)wq(0) * c0p, (R)wq(0) * c0m);
 #pragma unroll
-    for (int i = 0; i < Q; ++i) {
-        if (!((want >> i) & 1u)) continue;
-        const R e = feq_dev(i, phi[0], rphi, phi[1], phi[2], phi[3], usq);
-        store_ghost<T>(A, pidx(i, Lf.S, g), i,
-                       e + fac * (f[i] - feq_dev(i, mp[0], rp, mp[1], mp[2], mp[3], usp)), rb);
+    for (int k = 0; k < (Q - 1) / 2; ++k) {
+        const int i = 2 * k + 1;
+        const R cp = cdot(i, phi[1], phi[2], phi[3]), cm = cdot(i, mp[1], mp[2], mp[3]);
+        const R sp = (R)wq(i) * fmadd(r45p * cp, cp, c0p), ap = ((R)wq(i) * r3p) * cp;
+        const R sm = (R)wq(i) * fmadd(r45m * cm, cm, c0m), am = ((R)wq(i) * r3m) * cm;
+        put(i, sp + ap, sm + am);
+        put(i + 1, sp - ap, sm - am);
     }
 }
 
