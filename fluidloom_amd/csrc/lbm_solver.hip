@@ -803,6 +803,9 @@ uint64_t layout_bytes(const fl_lbm_solver* S) {
         t += std::max<uint64_t>((uint64_t)H.nactive * 4, (uint64_t)H.force_links.size() * 3) * 8;
         t += 2ull * H.nb * 64 * Q * sb + 2ull * cap_elems(H.cap_slots.size()) * cb;
         t += brick_map_bytes(H.nb);
+        // f2c staging: sums and flags per f2c ghost, the finer level's block map
+        if (!H.f2c_ghost.empty() && l + 1 < S->L)
+            t += H.f2c_ghost.size() * ((4 + Q) * cb + 1) + (uint64_t)S->h[l + 1].nb * 8 * sizeof(uint32_t);
     }
     return t;
 }
