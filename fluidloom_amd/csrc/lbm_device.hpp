@@ -406,7 +406,8 @@ struct LevelDev {
     const uint32_t* bflags;  // bit 0: simple brick (all 64 active, all links regular);
                              // bit 1: a cell of the brick has a pressure-outlet link;
                              // bit 2: a 2x2x2 block of the brick is the children of a
-                             // staged f2c ghost of the coarser level (f2c_blk)
+                             // staged f2c ghost of the coarser level (f2c_blk);
+                             // bit 3: a cell of the brick is a staged capture (cap_of)
     const uint64_t* cflags;  // [slot]
     const uint16_t* binst;   // [brick] instance id
     const float* itau;       // [instance] 1 / tau at this level
@@ -426,6 +427,14 @@ struct LevelDev {
     void* stage;
     uint32_t n_stage;
     int32_t stage_on;
+    // capture staging (this level's collide writing the capture that follows
+    // it): cap_of[slot] = the slot's capture index (kNoCap: none, or not
+    // staged); when cap_on, the collide writes those cells' pulled deviations
+    // and (rho - 1, momentum) into cap_out (ncap captures, the k_capture layout)
+    const uint32_t* cap_of;
+    void* cap_out;
+    uint32_t ncap;
+    int32_t cap_on;
 };
 
 __host__ __device__ inline double tau_at(double tau0, int level) {

@@ -141,7 +141,12 @@ __device__ __attribute__((always_inline)) inline void stage_children(const Level
 // position leaves the brick along each axis.  Offsets from a plane's base
 // are 32-bit (below 8 * slots, and the layout keeps slots below 2^27), on a
 // wave-uniform plane base.
-template <typename T, bool kOutlet>
+// kStaging: the f2c and capture staging code (LevelDev::stage_on, cap_on)
+// is compiled in.  The outlet instances leave it out, also on the path they
+// delegate plain bricks to: a second copy of it in one kernel body made the
+// 16-bit outlet instances allocate all 256 registers.  The solver never
+// asks an outlet level to stage.
+template <typename T, bool kOutlet, bool kStaging = !kOutlet>
 __device__ __attribute__((always_inline)) inline void collide_brick(const LevelDev& L, const T* __restrict__ A,
                                                                     T* __restrict__ B, int b_in) {
     using R = typename Store<T>::R;
@@ -164,7 +169,7 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
     if constexpr (kOutlet) {
         // only the bricks with an outlet link take the outlet code
         if (!(bf & 2u)) {
-            collide_brick<T, false>(L, A, B, b);
+            collide_brick<T, false, false>(L, A, B, b);
             return;
         }
     }
@@ -222,7 +227,24 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
 #pragma unroll
         for (int i = 0; i < Q; ++i) f[i] -= (R)wq(i);
     }
+    if constexpr (kStaging) {
     if (L.stage_on && (bf & 4u)) stage_children<R>(L, f, b, lane);
+    if (L.cap_on && (bf & 8u)) {
+        // the capture after this collide gathers these very values (k_capture
+        // on the collide's input); staged captures have regular links only,
+        // so the pulled values are the gather's bit for bit
+        const uint32_t kc = L.cap_of[slot];
+        if (kc != kNoCap) {
+            R m[4];
+            dev_density_momentum(f, m[0], m[1], m[2], m[3]);
+            R* __restrict__ cap = static_cast<R*>(L.cap_out);
+#pragma unroll
+            for (int i = 0; i < Q; ++i) cap[(size_t)i * L.ncap + kc] = f[i];
+            typedef R R4 __attribute__((ext_vector_type(4)));
+            *reinterpret_cast<R4*>(cap + cap_moments(L.ncap) + 4 * (size_t)kc) = R4{m[0], m[1], m[2], m[3]};
+        }
+    }
+    }
     bgk_dev(f, inv_tau_of<R>(L, inst));
     if constexpr (std::is_same<T, _Float16>::value || std::is_same<T, fp16c>::value) {
         // rounding bits from the cell's own coordinates, level and step, so an
@@ -320,13 +342,16 @@ __global__ void k_tick(uint32_t* tick) {
     if (threadIdx.x == 0) *tick += 1u;
 }
 
+// skip (optional): captures the collide already wrote (LevelDev::cap_on)
 template <typename T>
 __global__ __launch_bounds__(kWG) void k_capture(LevelDev L, const T* __restrict__ A,
                                                  const uint32_t* __restrict__ cap_slots, uint32_t ncap,
-                                                 typename Store<T>::R* __restrict__ cap) {
+                                                 typename Store<T>::R* __restrict__ cap,
+                                                 const uint8_t* __restrict__ skip) {
     using R = typename Store<T>::R;
     // deviations h = f - w and the record (rho - 1, momentum)
     for (uint32_t k = blockIdx.x * kWG + threadIdx.x; k < ncap; k += gridDim.x * kWG) {
+        if (skip && skip[k]) continue;
         R f[Q], m[4];
         gather_slot_dev<T, R>(L, A, cap_slots[k], f);
         dev_density_momentum(f, m[0], m[1], m[2], m[3]);
@@ -1085,10 +1110,39 @@ hipError_t launch_tick(uint32_t* tick, hipStream_t s) {
 }
 
 template <typename T>
-hipError_t launch_capture(const LevelDev& L, const T* A, const uint32_t* slots, uint32_t ncap, void* cap, hipStream_t s) {
+hipError_t launch_capture(const LevelDev& L, const T* A, const uint32_t* slots, uint32_t ncap, void* cap,
+                          const uint8_t* skip, hipStream_t s) {
     if (ncap == 0) return hipSuccess;
     hipLaunchKernelGGL(k_capture<T>, dim3(lgrid(ncap)), dim3(kWG), 0, s, L, A, slots, ncap,
-                       static_cast<typename Store<T>::R*>(cap));
+                       static_cast<typename Store<T>::R*>(cap), skip);
+    return hipGetLastError();
+}
+
+// Capture staging map of a level: capture k is staged when its slot is an
+// active fluid leaf whose links are all regular pulls (the collide's pulled
+// values are then the capture's gather bit for bit); cap_of[slot] = k and
+// the brick's bflags gain bit 3.  cap_of starts all kNoCap.
+__global__ __launch_bounds__(kWG) void k_cap_map(const uint32_t* __restrict__ slots, uint32_t ncap,
+                                                 const uint64_t* __restrict__ cflags, uint32_t* __restrict__ bflags,
+                                                 uint32_t* __restrict__ cap_of, uint8_t* __restrict__ staged) {
+    const uint32_t k = blockIdx.x * kWG + threadIdx.x;
+    if (k >= ncap) return;
+    const uint32_t sl = slots[k];
+    const uint64_t w = cflags[sl];
+    const bool ok = (w & kActiveBit) != 0 && (w & ((1ull << (2 * Q)) - 1)) == 0;
+    staged[k] = ok ? 1 : 0;
+    if (!ok) return;
+    cap_of[sl] = k;
+    atomicOr(bflags + (sl >> 6), 8u);
+}
+
+hipError_t launch_cap_map(const uint32_t* slots, uint32_t ncap, const uint64_t* cflags, uint32_t* bflags,
+                          uint32_t* cap_of, size_t nslots, uint8_t* staged, hipStream_t s) {
+    if (nslots)
+        hipLaunchKernelGGL(k_fill_u32_kernel, dim3((unsigned)((nslots + kWG - 1) / kWG)), dim3(kWG), 0, s, cap_of,
+                           nslots, kNoCap);
+    if (ncap) hipLaunchKernelGGL(k_cap_map, dim3((ncap + kWG - 1) / kWG), dim3(kWG), 0, s, slots, ncap, cflags, bflags,
+                                 cap_of, staged);
     return hipGetLastError();
 }
 
@@ -1266,7 +1320,8 @@ hipError_t launch_ghost_eq(const uint32_t* c2f_gchild, const uint32_t* c2f_gcap,
 
 #define FL_INST(T)                                                                                            \
     template hipError_t launch_collide<T>(const LevelDev&, const T*, T*, int, hipStream_t);                    \
-    template hipError_t launch_capture<T>(const LevelDev&, const T*, const uint32_t*, uint32_t, void*, hipStream_t); \
+    template hipError_t launch_capture<T>(const LevelDev&, const T*, const uint32_t*, uint32_t, void*,            \
+                                          const uint8_t*, hipStream_t);                                          \
     template hipError_t launch_fill_c2f<T>(const LevelDev&, const uint32_t*, const uint32_t*, const uint32_t*,  \
                                            uint32_t, const void*, const void*, uint32_t, double, T*, hipStream_t); \
     template hipError_t launch_fill_f2c<T>(const LevelDev&, const uint32_t*, const uint32_t*, uint32_t,         \

@@ -23,7 +23,11 @@ hipError_t launch_collide(const LevelDev& L, const T* A, T* B, int variant, hipS
 hipError_t launch_any_outlet(const uint64_t* cflags, uint32_t* bflags, size_t nslots, uint32_t* flag, hipStream_t s);
 hipError_t launch_tick(uint32_t* tick, hipStream_t s);
 template <typename T>
-hipError_t launch_capture(const LevelDev& L, const T* A, const uint32_t* slots, uint32_t ncap, void* cap, hipStream_t s);
+hipError_t launch_capture(const LevelDev& L, const T* A, const uint32_t* slots, uint32_t ncap, void* cap,
+                          const uint8_t* skip, hipStream_t s);
+// capture staging map of a level (nslots slots) for its ncap captures (k_cap_map)
+hipError_t launch_cap_map(const uint32_t* slots, uint32_t ncap, const uint64_t* cflags, uint32_t* bflags,
+                          uint32_t* cap_of, size_t nslots, uint8_t* staged, hipStream_t s);
 template <typename T>
 hipError_t launch_fill_c2f(const LevelDev& Lf, const uint32_t* gchild, const uint32_t* gcap, const uint32_t* gmask,
                            uint32_t ngroups, const void* cap0, const void* cap1, uint32_t ncap, double alpha, T* A,

@@ -224,6 +224,10 @@ struct DevLevel {
     uint8_t* f2c_staged = nullptr;
     uint32_t* f2c_blk = nullptr;
     bool stage_fresh = false;
+    // capture staging: slot -> capture index of the captures this level's
+    // collide writes itself (LevelDev::cap_of), and which captures those are
+    uint32_t* cap_of = nullptr;
+    uint8_t* cap_staged = nullptr;
     int cur = 0;
     LevelDev ld{};
 };
@@ -806,6 +810,8 @@ uint64_t layout_bytes(const fl_lbm_solver* S) {
         // f2c staging: sums and flags per f2c ghost, the finer level's block map
         if (!H.f2c_ghost.empty() && l + 1 < S->L)
             t += H.f2c_ghost.size() * ((4 + Q) * cb + 1) + (uint64_t)S->h[l + 1].nb * 8 * sizeof(uint32_t);
+        // capture staging: slot -> capture map and flags
+        if (!H.cap_slots.empty()) t += (uint64_t)H.nb * 64 * sizeof(uint32_t) + H.cap_slots.size();
     }
     return t;
 }
@@ -924,6 +930,20 @@ fl_status upload_and_init(fl_lbm_solver* S, bool tables_on_device, bool remap_fo
         F.ld.n_stage = (uint32_t)H.n_f2c;
         F.ld.stage_on = 0;
     }
+    // capture staging: which of a level's captures its own collide can write
+    for (int l = 0; l < S->L; ++l) {
+        const HostLevel& H = S->h[l];
+        DevLevel& D = S->d[l];
+        if (H.n_cap == 0) continue;
+        const size_t ns = (size_t)H.nb * 64;
+        if ((r = dev_alloc(S, reinterpret_cast<void**>(&D.cap_of), ns * sizeof(uint32_t)))) return r;
+        if ((r = dev_alloc(S, reinterpret_cast<void**>(&D.cap_staged), H.n_cap))) return r;
+        FL_HIP(launch_cap_map(D.cap_slots, (uint32_t)H.n_cap, D.cflags, D.bflags, D.cap_of, ns, D.cap_staged,
+                              S->stream));
+        D.ld.cap_of = D.cap_of;
+        D.ld.ncap = (uint32_t)H.n_cap;
+        D.ld.cap_on = 0;
+    }
     return FL_OK;
 }
 
@@ -954,16 +974,24 @@ fl_status advance(fl_lbm_solver* S, int l, bool stage_sub) {
         // levels write only their own buffers).  So within a pass only the
         // first step captures it; later steps take the previous capture.
         if (!D.cap_fresh)
-            FL_HIP(launch_capture<T>(D.ld, static_cast<const T*>(D.buf[1 - D.cur]), D.cap_slots, ncap, D.cap[0], st));
+            FL_HIP(launch_capture<T>(D.ld, static_cast<const T*>(D.buf[1 - D.cur]), D.cap_slots, ncap, D.cap[0],
+                                     nullptr, st));
     }
+    // staging: the collide writes the staged f2c sums of the coarser level
+    // (second sub-step) and the staged captures that follow it; levels with
+    // outlet links run the outlet instances, which have no staging code
+    const bool cap_stage = fine && D.cap_of && !D.ld.outlet;
     LevelDev ld = D.ld;
-    ld.stage_on = stage_sub && ld.stage ? 1 : 0;
+    ld.stage_on = stage_sub && ld.stage && !ld.outlet ? 1 : 0;
+    ld.cap_on = cap_stage ? 1 : 0;
+    ld.cap_out = D.cap[1];
     FL_HIP(launch_collide<T>(ld, static_cast<const T*>(D.buf[D.cur]), static_cast<T*>(D.buf[1 - D.cur]),
                              S->variant[l], st));
     if (S->storage == FL_STORE_F16 || S->storage == FL_STORE_F16C) FL_HIP(launch_tick(S->ticks + l, st));
     D.cur ^= 1;
     if (fine) {
-        FL_HIP(launch_capture<T>(D.ld, static_cast<const T*>(D.buf[1 - D.cur]), D.cap_slots, ncap, D.cap[1], st));
+        FL_HIP(launch_capture<T>(D.ld, static_cast<const T*>(D.buf[1 - D.cur]), D.cap_slots, ncap, D.cap[1],
+                                 cap_stage ? D.cap_staged : nullptr, st));
         DevLevel& F = S->d[l + 1];
         const HostLevel& HF = S->h[l + 1];
         for (int sub = 0; sub < 2; ++sub) {
@@ -975,7 +1003,7 @@ fl_status advance(fl_lbm_solver* S, int l, bool stage_sub) {
         // this step's capture after the collide is the next step's before it
         std::swap(D.cap[0], D.cap[1]);
         D.cap_fresh = true;
-        D.stage_fresh = D.f2c_stage != nullptr;
+        D.stage_fresh = D.f2c_stage != nullptr && !S->d[l + 1].ld.outlet;
     }
     return FL_OK;
 }
