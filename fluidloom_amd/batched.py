@@ -120,7 +120,7 @@ def plain_rate(specs, mesh_list, steps, sample, tuned=False):
     return upd / t / 1e6, upd, t
 
 
-def run(instances=256, steps=2000, regrid_every=0, plain_sample=32, plain_steps=200, seed=42, nmax=96,
+def run(instances=256, steps=2000, regrid_every=0, plain_sample=256, plain_steps=200, seed=42, nmax=96,
         forms=("f32", "f16", "f16c")):
     check(lib().fl_init(0))
     specs = instance_specs(instances, seed, nmax=nmax)
@@ -208,7 +208,7 @@ def main():
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--regrid-every", type=int, default=0)
-    ap.add_argument("--plain-sample", type=int, default=32)
+    ap.add_argument("--plain-sample", type=int, default=256)
     ap.add_argument("--plain-steps", type=int, default=200)
     ap.add_argument("--nmax", type=int, default=96)
     ap.add_argument("--seed", type=int, default=42)
