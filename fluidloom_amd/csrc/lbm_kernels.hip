@@ -155,8 +155,9 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
     // What the population loads need comes from b alone: the neighbour row and
     // the brick flags are one round of scalar loads.  The instance (its live
     // flag and relaxation rate) only matters to the collision and the stores,
-    // so its two dependent loads overlap the population loads; a stopped
-    // instance keeps its layout, so reading its populations stays in bounds.
+    // so its two dependent loads overlap the population loads, and the live
+    // flag is tested after them; a stopped instance keeps its layout, so
+    // reading its populations stays in bounds.
     // binst is 16-bit: the wave-uniform dword holding it keeps the load on
     // the scalar unit (device allocations are whole multiples of 256 B, so
     // the dword's other half is inside the array).
@@ -174,6 +175,25 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
         }
     }
     const int inst = (int)((b & 1) ? ipair >> 16 : ipair & 0xFFFFu);
+    // The instance's live flag and relaxation rate, and in 16-bit storage the
+    // brick's rounding hash, are read here, ahead of every store, so they are
+    // scalar loads issued before the population loads.  Read where they are
+    // used, after a possible staging store, they were vector loads issued
+    // only once the populations had arrived.
+    const int live = L.inst[inst].live;
+    const R itau = inv_tau_of<R>(L, inst);
+    constexpr bool k16 = std::is_same<T, _Float16>::value || std::is_same<T, fp16c>::value;
+    uint32_t hb = 0;
+    if constexpr (k16) {
+        // rounding bits from the cell's own coordinates, level and step, so an
+        // instance rounds the same way alone and inside any batch: the brick's
+        // coordinates, level and step hashed once per wave (wave-uniform:
+        // scalar-unit multiplies)
+        const int32_t* c3 = L.bcoord + 3 * b;
+        hb = mix32((uint32_t)c3[0] * 0x9E3779B1u ^
+                   mix32((uint32_t)c3[1] * 0x85EBCA77u ^
+                         mix32((uint32_t)c3[2] * 0xC2B2AE3Du ^ ((uint32_t)L.level << 24) ^ (*L.tick * 0x27D4EB2Fu))));
+    }
     const int lane = (int)(threadIdx.x & 63);
     const bool simple = (bf & 1u) != 0;
     const unsigned slot = (unsigned)b * 64u + (unsigned)lane;
@@ -222,7 +242,7 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
             }
         }
     }
-    if (!active || !L.inst[inst].live) return;
+    if (!active || !live) return;
     if (!kDevLoads) {
 #pragma unroll
         for (int i = 0; i < Q; ++i) f[i] -= (R)wq(i);
@@ -245,22 +265,14 @@ __device__ __attribute__((always_inline)) inline void collide_brick(const LevelD
         }
     }
     }
-    bgk_dev(f, inv_tau_of<R>(L, inst));
-    if constexpr (std::is_same<T, _Float16>::value || std::is_same<T, fp16c>::value) {
-        // rounding bits from the cell's own coordinates, level and step, so an
-        // instance rounds the same way alone and inside any batch.  The
-        // brick's coordinates, level and step are hashed once per wave (all
-        // wave-uniform: scalar-unit multiplies), then one hash of that plus
-        // the lane gives the cell's h.  Direction i draws h + i * 0x9E3779B9
-        // (a Weyl sequence: one add per direction, where the kernel is bound
-        // by VALU issue in 16-bit storage).  Each draw is as uniform over
-        // cells and steps as h, so each direction's rounding stays unbiased;
-        // the 19 draws of one cell are spread evenly over the 32-bit range.
-        const int32_t* c3 = L.bcoord + 3 * b;
-        const uint32_t hb = mix32((uint32_t)c3[0] * 0x9E3779B1u ^
-                                  mix32((uint32_t)c3[1] * 0x85EBCA77u ^
-                                        mix32((uint32_t)c3[2] * 0xC2B2AE3Du ^ ((uint32_t)L.level << 24) ^
-                                              (*L.tick * 0x27D4EB2Fu))));
+    bgk_dev(f, itau);
+    if constexpr (k16) {
+        // One hash of the brick's hash hb plus the lane gives the cell's h.
+        // Direction i draws h + i * 0x9E3779B9 (a Weyl sequence: one add per
+        // direction, where the kernel is bound by VALU issue in 16-bit
+        // storage).  Each draw is as uniform over cells and steps as h, so
+        // each direction's rounding stays unbiased; the 19 draws of one cell
+        // are spread evenly over the 32-bit range.
         const uint32_t h = mix32(hb + (uint32_t)lane);
         if constexpr (std::is_same<T, _Float16>::value) {
             // v_cvt_sr_f16_f32 writes one half of its register and keeps the
