@@ -94,3 +94,42 @@ def test_sphere_quarter_scale_f32_matches_oracle(fl, oracle):
           f"force {e_f:.2e} (Fx {Fg[0]:.6e} vs {Fo[0]:.6e})", flush=True)
     g.close()
     assert e_rho < 1e-4 and e_u < 1e-4 and e_f < 1e-4
+
+
+def test_sphere_full_size_f32_matches_fp64_run(fl):
+    """The sphere configuration at its full size (base 256x128x128, r = 16,
+    3 levels within 2r, Re 100; 76.4M leaves), FP32 storage against the
+    FP64-storage run of the same kernels (the GPU oracle run, held to the
+    CPU oracle within 1e-11 on the smaller cases), 200 coarse steps (1,600
+    level-3 sub-steps) from the same start: density and velocity within 1e-4
+    relative L2 over every leaf, and the drag coefficient within 1e-4
+    relative.  The same comparison over the configuration's 5,000 steps is
+    `python -m fluidloom_amd.sphere --compare f32`."""
+    from fluidloom_amd import sphere
+
+    print("full-size sphere: FP32 and FP64 runs, 200 steps", flush=True)
+    out = sphere.precision(steps=200, report_every=100, forms=("f32",), tune=False)
+    r = out["results"]["f32"]
+    print(f"full-size sphere FP32 vs FP64, 200 steps: rho {r['rho_l2']:.2e} u {r['vel_l2']:.2e} "
+          f"C_d {r['cd']:.6f} vs {out['results']['f64']['cd']:.6f} (rel {r['cd_rel_err']:.2e})", flush=True)
+    assert out["leaves"] > 76_000_000
+    assert r["rho_l2"] < 1e-4 and r["vel_l2"] < 1e-4 and r["cd_rel_err"] < 1e-4
+
+
+def test_sphere_precision_driver_small(fl):
+    """The storage-form comparison on a small sphere (base 32x16x16, r = 2,
+    3 levels, 40 steps): every form reports; FP32 stays within the FP32
+    tolerance of the FP64 run, and the 16-bit forms within 1e-2 (about 25
+    times the 3.8e-4 / 1.8e-4 they show at twice this size after 200
+    steps); each form's mass drift matches the FP64 run's to 1e-4."""
+    from fluidloom_amd import sphere
+
+    out = sphere.precision(steps=40, report_every=20, scale=8, forms=("f32", "f16", "f16c"), tune=False)
+    res = out["results"]
+    assert set(res) == {"f64", "f32", "f16", "f16c"}
+    assert len(out["history"]) == 2 and all(np.isfinite(v) for row in out["history"] for v in row.values())
+    assert res["f32"]["vel_l2"] < 1e-4 and res["f32"]["rho_l2"] < 1e-4 and res["f32"]["cd_rel_err"] < 1e-4
+    for st in ("f16", "f16c"):
+        assert res[st]["vel_l2"] < 1e-2 and res[st]["rho_l2"] < 1e-3, (st, res[st])
+    for st in ("f32", "f16", "f16c"):
+        assert abs(res[st]["mass_drift"] - res["f64"]["mass_drift"]) < 1e-4, (st, res[st])
